@@ -1,0 +1,788 @@
+// eimp_engine.cpp -- host side of the engine behind the C ABI in include/eimp.h.
+//
+// One engine = one device, one HIP stream, the resident panel and grow-only
+// device pools for the per-batch data.  A batch runs in two phases:
+//   A  (prep)   distances -> top-K -> monos -> split/call sites
+//   B  (build)  condensed reference -> forward/backward PBWT (+ digests)
+// with one small device->host read of the split counts in between, which
+// sizes the variable-length outputs of phase B.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/eimp.h"
+#include "eimp_internal.h"
+
+namespace eimp {
+hipError_t launch_site_to_hap(const PanelDev &P, uint64_t *hap, hipStream_t s);
+hipError_t launch_panel_mono(const PanelDev &P, uint64_t *pand, uint64_t *por, hipStream_t s);
+hipError_t launch_distance(const PanelDev &P, const TgtDev *tg, const uint32_t *tlist,
+                           uint32_t nsel, hipStream_t s);
+hipError_t launch_select(const PanelDev &P, const TgtDev *tg, uint32_t n, hipStream_t s);
+hipError_t launch_monos(const PanelDev &P, const TgtDev *tg, const uint32_t *tlist, uint32_t n,
+                        uint32_t max_ww, hipStream_t s);
+hipError_t launch_splits(const PanelDev &P, const TgtDev *tg, uint32_t n, hipStream_t s);
+hipError_t launch_bits_to_list(const TgtDev *tg, uint32_t n, uint32_t nbits_bh, int which,
+                               hipStream_t s);
+hipError_t launch_cref(const PanelDev &P, const TgtDev *tg, const CrefItem *items, uint32_t nitems,
+                       uint32_t maxK, hipStream_t s);
+uint32_t cref_max_k();
+hipError_t launch_digest_count0(const TgtDev *tg, uint32_t n, hipStream_t s);
+hipError_t launch_cref_rows(const TgtDev *tg, uint32_t t, uint32_t Mp, uint64_t *rows, uint32_t rw,
+                            hipStream_t s);
+int pbwt_variant_count();
+int pbwt_variant_threads(int v);
+int pbwt_variant_cpw(int v);
+uint32_t pbwt_variant_maxk(int v);
+size_t pbwt_lds_bytes(int v, uint32_t Kpad);
+hipError_t pbwt_prepare(int v, size_t lds);
+hipError_t launch_pbwt(int v, const TgtDev *tg, const PbwtItem *items, uint32_t nitems, size_t lds,
+                       hipStream_t s);
+}  // namespace eimp
+
+using namespace eimp;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                         \
+    do {                                                                                  \
+        hipError_t e_ = (x);                                                              \
+        if (e_ != hipSuccess)                                                             \
+            return fail(EIMP_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+#define CHK(x)                      \
+    do {                            \
+        int r_ = (x);               \
+        if (r_ != EIMP_OK) return r_; \
+    } while (0)
+
+inline uint64_t unit_words(uint64_t nbits) { return ((nbits + 511) / 512) * 8; }
+inline uint64_t round_up(uint64_t n, uint64_t m) { return (n + m - 1) / m * m; }
+
+// Grow-only device buffer
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    int ensure(size_t bytes, const char *what) {
+        if (bytes <= n && p) return EIMP_OK;
+        release();
+        size_t want = std::max<size_t>(bytes + bytes / 8, 256);
+        if (hipMalloc(&p, want) != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return fail(EIMP_ERR_NOMEM, std::string("device allocation failed: ") + what + " (" +
+                                            std::to_string(want) + " bytes)");
+        }
+        n = want;
+        return EIMP_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    template <typename T>
+    T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+// Grow-only pinned host buffer
+struct HostBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    int ensure(size_t bytes) {
+        if (bytes <= n && p) return EIMP_OK;
+        release();
+        size_t want = std::max<size_t>(bytes + bytes / 8, 256);
+        if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return fail(EIMP_ERR_NOMEM, "pinned host allocation failed");
+        }
+        n = want;
+        return EIMP_OK;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    template <typename T>
+    T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+struct PendingEvent {
+    int cls;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct eimp_engine {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    bool initialized = false;
+    // reference GPUEngine parameters
+    size_t K = 0, Kwords = 0, M = 0, Ncorr = 0, Nad = 0, refTCap = 0;
+    // resident panel
+    DevBuf site, hap, cM, haploid, pand, por;
+    bool have_cM = false, have_hap = false, have_pmono = false;
+    uint64_t hap_stride = 0;
+    PanelDev pd{};
+    // batch pools
+    DevBuf d_tg, d_gt, d_nerr, d_bh, d_bhflags, d_mono, d_splitsites, d_splits, d_cMdiffs,
+        d_stats, d_cref, d_count0, d_out, d_digest, d_tlist, d_citems, d_pitems, d_tmp;
+    HostBuf h_stage;
+    std::vector<TgtDev> h_tg;
+    std::vector<uint32_t> h_stats;
+    uint32_t last_n = 0;
+    // timing
+    bool timing = false;
+    double ms[EIMP_NKCLASS] = {};
+    uint64_t launches[EIMP_NKCLASS] = {};
+    std::vector<PendingEvent> pending;
+    std::vector<hipEvent_t> event_pool;
+    int pbwt_variant = -1;
+    int pbwt_prepared_variant = -1;
+    size_t pbwt_prepared_lds = 0;
+
+    hipEvent_t get_event() {
+        if (!event_pool.empty()) {
+            hipEvent_t e = event_pool.back();
+            event_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    // Records a start event for a kernel class; returns the index to close.
+    int tbegin(int cls) {
+        if (!timing) return -1;
+        PendingEvent pe{cls, get_event(), get_event()};
+        if (!pe.a || !pe.b) return -1;
+        (void)hipEventRecord(pe.a, stream);
+        pending.push_back(pe);
+        return (int)pending.size() - 1;
+    }
+    void tend(int idx) {
+        if (idx < 0) return;
+        (void)hipEventRecord(pending[idx].b, stream);
+        launches[pending[idx].cls]++;
+    }
+    // after a stream synchronisation
+    void tcollect() {
+        for (auto &pe : pending) {
+            float t = 0;
+            if (hipEventElapsedTime(&t, pe.a, pe.b) == hipSuccess) ms[pe.cls] += t;
+            event_pool.push_back(pe.a);
+            event_pool.push_back(pe.b);
+        }
+        pending.clear();
+    }
+    int sync() {
+        HIPCHK(hipStreamSynchronize(stream));
+        tcollect();
+        return EIMP_OK;
+    }
+    void release_pools() {
+        DevBuf *bufs[] = {&d_tg, &d_gt, &d_nerr, &d_bh, &d_bhflags, &d_mono, &d_splitsites,
+                          &d_splits, &d_cMdiffs, &d_stats, &d_cref, &d_count0, &d_out,
+                          &d_digest, &d_tlist, &d_citems, &d_pitems, &d_tmp};
+        for (auto *b : bufs) b->release();
+        h_stage.release();
+    }
+    void release_panel() {
+        site.release();
+        hap.release();
+        cM.release();
+        haploid.release();
+        pand.release();
+        por.release();
+        have_cM = have_hap = have_pmono = false;
+    }
+    int choose_pbwt_variant(uint32_t Kmax_pad) {
+        if (pbwt_variant >= 0) {
+            if (pbwt_variant >= pbwt_variant_count()) return fail(EIMP_ERR_INVALID, "bad variant");
+            if (pbwt_variant_maxk(pbwt_variant) < Kmax_pad)
+                return fail(EIMP_ERR_UNSUPPORTED, "PBWT variant too small for K");
+            return pbwt_variant;
+        }
+        // smallest chunk count per wave that covers K with 256 threads, else
+        // larger workgroups
+        int best = -1;
+        for (int v = 0; v < pbwt_variant_count(); v++) {
+            if (pbwt_variant_maxk(v) < Kmax_pad) continue;
+            if (best < 0) { best = v; continue; }
+            // prefer fewer threads, then fewer chunks per wave
+            int bt = pbwt_variant_threads(best), vt = pbwt_variant_threads(v);
+            if (vt < bt || (vt == bt && pbwt_variant_cpw(v) < pbwt_variant_cpw(best))) best = v;
+        }
+        if (best < 0) return fail(EIMP_ERR_UNSUPPORTED, "K beyond the PBWT kernels' LDS capacity");
+        return best;
+    }
+};
+
+namespace {
+
+int set_device(eimp_engine *e) {
+    HIPCHK(hipSetDevice(e->dev));
+    return EIMP_OK;
+}
+
+int ensure_hap(eimp_engine *e) {
+    if (e->have_hap) return EIMP_OK;
+    e->hap_stride = unit_words(e->M);
+    CHK(e->hap.ensure(e->Nad * e->hap_stride * 8, "hap-major panel"));
+    HIPCHK(hipMemsetAsync(e->hap.p, 0, e->Nad * e->hap_stride * 8, e->stream));
+    e->pd.hap = e->hap.as<uint64_t>();
+    e->pd.hap_stride = e->hap_stride;
+    HIPCHK(launch_site_to_hap(e->pd, e->hap.as<uint64_t>(), e->stream));
+    e->have_hap = true;
+    return e->sync();
+}
+
+int ensure_pmono(eimp_engine *e) {
+    if (e->have_pmono) return EIMP_OK;
+    size_t words = unit_words(e->M);
+    CHK(e->pand.ensure(words * 8, "panel mono"));
+    CHK(e->por.ensure(words * 8, "panel mono"));
+    HIPCHK(hipMemsetAsync(e->pand.p, 0, words * 8, e->stream));
+    HIPCHK(hipMemsetAsync(e->por.p, 0, words * 8, e->stream));
+    HIPCHK(launch_panel_mono(e->pd, e->pand.as<uint64_t>(), e->por.as<uint64_t>(), e->stream));
+    e->pd.pand = e->pand.as<uint64_t>();
+    e->pd.por = e->por.as<uint64_t>();
+    e->have_pmono = true;
+    return e->sync();
+}
+
+// Phase B for targets whose phase-A fields (bh, splitsites, S) are in place.
+// Sizes outputs, builds cref + PBWT.  rs_mode: 0 GPU layout, 1 compact.
+int run_build(eimp_engine *e, uint32_t n, bool digest, bool compact) {
+    std::vector<TgtDev> &tg = e->h_tg;
+    uint64_t cref_bytes = 0, c0_bytes = 0, out_bytes = 0;
+    std::vector<uint64_t> cref_off(n), c0_off(n), out_off(n);
+    uint32_t Kmax_pad = 0, Kmax = 0;
+    std::vector<CrefItem> citems;
+    std::vector<PbwtItem> pitems;
+    const uint32_t bpi = 2;  // 16-row blocks per cref work item
+    for (uint32_t t = 0; t < n; t++) {
+        TgtDev &T = tg[t];
+        T.Mp = 2 * T.S + 1;
+        T.Kpad = (uint32_t)round_up(T.K, 64);
+        T.RS = compact ? (uint32_t)(2 * ((T.K + 31) / 32)) : (uint32_t)(4 * unit_words(T.K));
+        const uint64_t nblk = (T.Mp + 15) / 16;
+        cref_off[t] = cref_bytes;
+        cref_bytes += round_up(nblk * T.Kpad * 2, 256);
+        c0_off[t] = c0_bytes;
+        c0_bytes += round_up((uint64_t)T.Mp * 4, 256);
+        out_off[t] = out_bytes;
+        out_bytes += round_up(2ull * T.Mp * T.RS * 4, 256);
+        Kmax_pad = std::max(Kmax_pad, T.Kpad);
+        Kmax = std::max(Kmax, T.K);
+        for (uint32_t b = 0; b < nblk; b += bpi)
+            citems.push_back(CrefItem{t, b, (uint32_t)std::min<uint64_t>(b + bpi, nblk), 0});
+        pitems.push_back(PbwtItem{t, 0});
+        pitems.push_back(PbwtItem{t, 1});
+    }
+    if (Kmax > cref_max_k()) return fail(EIMP_ERR_UNSUPPORTED, "K beyond the cref kernel limit");
+    int v = e->choose_pbwt_variant(Kmax_pad);
+    if (v < 0) return v;
+    const size_t lds = pbwt_lds_bytes(v, Kmax_pad);
+    if (lds > 160 * 1024) return fail(EIMP_ERR_UNSUPPORTED, "PBWT LDS need exceeds 160 KiB");
+    if (e->pbwt_prepared_variant != v || e->pbwt_prepared_lds < lds) {
+        HIPCHK(pbwt_prepare(v, lds));
+        e->pbwt_prepared_variant = v;
+        e->pbwt_prepared_lds = lds;
+    }
+    // longest targets first
+    std::stable_sort(pitems.begin(), pitems.end(), [&](const PbwtItem &a, const PbwtItem &b) {
+        return tg[a.t].Mp > tg[b.t].Mp;
+    });
+    CHK(e->d_cref.ensure(cref_bytes, "cref"));
+    CHK(e->d_count0.ensure(c0_bytes, "count0"));
+    CHK(e->d_out.ensure(out_bytes, "cPBWT output"));
+    CHK(e->d_digest.ensure((size_t)n * 16, "digest"));
+    CHK(e->d_citems.ensure(citems.size() * sizeof(CrefItem), "cref items"));
+    CHK(e->d_pitems.ensure(pitems.size() * sizeof(PbwtItem), "pbwt items"));
+    for (uint32_t t = 0; t < n; t++) {
+        TgtDev &T = tg[t];
+        T.cref = reinterpret_cast<uint16_t *>(e->d_cref.as<char>() + cref_off[t]);
+        T.count0 = reinterpret_cast<int32_t *>(e->d_count0.as<char>() + c0_off[t]);
+        T.out = reinterpret_cast<uint32_t *>(e->d_out.as<char>() + out_off[t]);
+        T.digest = digest ? e->d_digest.as<uint64_t>() + 2 * t : nullptr;
+    }
+    if (digest) HIPCHK(hipMemsetAsync(e->d_digest.p, 0, (size_t)n * 16, e->stream));
+    HIPCHK(hipMemcpyAsync(e->d_tg.p, tg.data(), n * sizeof(TgtDev), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->d_citems.p, citems.data(), citems.size() * sizeof(CrefItem),
+                          hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->d_pitems.p, pitems.data(), pitems.size() * sizeof(PbwtItem),
+                          hipMemcpyHostToDevice, e->stream));
+    int ti = e->tbegin(EIMP_KCLASS_CREF);
+    HIPCHK(launch_cref(e->pd, e->d_tg.as<TgtDev>(), e->d_citems.as<CrefItem>(),
+                       (uint32_t)citems.size(), Kmax, e->stream));
+    e->tend(ti);
+    ti = e->tbegin(EIMP_KCLASS_PBWT);
+    HIPCHK(launch_pbwt(v, e->d_tg.as<TgtDev>(), e->d_pitems.as<PbwtItem>(), (uint32_t)pitems.size(),
+                       lds, e->stream));
+    e->tend(ti);
+    if (digest) HIPCHK(launch_digest_count0(e->d_tg.as<TgtDev>(), n, e->stream));
+    return EIMP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *eimp_last_error(void) { return g_err.c_str(); }
+
+int eimp_device_count(int *count) {
+    if (!count) return fail(EIMP_ERR_INVALID, "null count");
+    int c = 0;
+    hipError_t err = hipGetDeviceCount(&c);
+    if (err != hipSuccess) {
+        *count = 0;
+        (void)hipGetLastError();
+        return fail(EIMP_ERR_HIP, std::string("hipGetDeviceCount: ") + hipGetErrorString(err));
+    }
+    *count = c;
+    return EIMP_OK;
+}
+
+int eimp_engine_create(int device_index, eimp_engine **engine) {
+    if (!engine) return fail(EIMP_ERR_INVALID, "null engine pointer");
+    *engine = nullptr;
+    int c = 0;
+    CHK(eimp_device_count(&c));
+    if (device_index < 0 || device_index >= c)
+        return fail(EIMP_ERR_INVALID, "device index " + std::to_string(device_index) +
+                                          " out of range (" + std::to_string(c) + " devices)");
+    eimp_engine *e = new eimp_engine();
+    e->dev = device_index;
+    if (set_device(e) != EIMP_OK || hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete e;
+        return fail(EIMP_ERR_HIP, "stream creation failed");
+    }
+    *engine = e;
+    return EIMP_OK;
+}
+
+int eimp_engine_free(eimp_engine *e) {
+    if (!e) return fail(EIMP_ERR_INVALID, "null engine");
+    CHK(set_device(e));
+    (void)hipStreamSynchronize(e->stream);
+    e->tcollect();
+    e->release_pools();
+    e->release_panel();
+    e->initialized = false;
+    return EIMP_OK;
+}
+
+int eimp_engine_destroy(eimp_engine *e) {
+    if (!e) return EIMP_OK;
+    eimp_engine_free(e);
+    for (auto ev : e->event_pool) (void)hipEventDestroy(ev);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return EIMP_OK;
+}
+
+int eimp_engine_initialize(eimp_engine *e, size_t K, size_t Kwords, size_t M, size_t Nrefhapscorr,
+                           size_t NrefhapsAD, size_t refTCapacity, const uint64_t *refTData) {
+    if (!e) return fail(EIMP_ERR_INVALID, "null engine");
+    if (!refTData || M == 0 || NrefhapsAD == 0) return fail(EIMP_ERR_INVALID, "empty panel");
+    if (Nrefhapscorr > NrefhapsAD) return fail(EIMP_ERR_INVALID, "Nrefhapscorr > NrefhapsAD");
+    if (K < 1 || K > Nrefhapscorr) return fail(EIMP_ERR_INVALID, "K must be in [1, Nrefhapscorr]");
+    if (Kwords != unit_words(K))
+        return fail(EIMP_ERR_INVALID, "Kwords must be roundToMultiple(K,512)/64");
+    if (refTCapacity % 64 || refTCapacity * 8 < NrefhapsAD)
+        return fail(EIMP_ERR_INVALID, "refTCapacity must be a multiple of 64 bytes holding NrefhapsAD bits");
+    if (NrefhapsAD >= (1u << 31) || M >= (1u << 31)) return fail(EIMP_ERR_UNSUPPORTED, "panel too large");
+    CHK(set_device(e));
+    if (e->initialized) {
+        (void)hipStreamSynchronize(e->stream);
+        e->release_panel();
+    }
+    e->K = K;
+    e->Kwords = Kwords;
+    e->M = M;
+    e->Ncorr = Nrefhapscorr;
+    e->Nad = NrefhapsAD;
+    e->refTCap = refTCapacity;
+    const size_t bytes = M * refTCapacity;
+    CHK(e->site.ensure(bytes, "site-major panel"));
+    HIPCHK(hipMemcpy(e->site.p, refTData, bytes, hipMemcpyHostToDevice));
+    e->pd = PanelDev{};
+    e->pd.site = e->site.as<uint64_t>();
+    e->pd.site_stride = refTCapacity / 8;
+    e->pd.N_ad = (uint32_t)NrefhapsAD;
+    e->pd.N_corr = (uint32_t)Nrefhapscorr;
+    e->pd.M = (uint32_t)M;
+    e->initialized = true;
+    return EIMP_OK;
+}
+
+int eimp_engine_block_size(eimp_engine *e, size_t *bs) {
+    if (!e || !bs) return fail(EIMP_ERR_INVALID, "null argument");
+    // Targets per call: the reference uses 256 (src/GPUEngine.cu:66); the
+    // engine takes any number, 256 keeps the drop-in's host buffers bounded.
+    *bs = 256;
+    return EIMP_OK;
+}
+
+int eimp_engine_device_index(eimp_engine *e, int *index) {
+    if (!e || !index) return fail(EIMP_ERR_INVALID, "null argument");
+    *index = e->dev;
+    return EIMP_OK;
+}
+
+int eimp_engine_set_panel_extras(eimp_engine *e, const float *cM, const uint64_t *hapData,
+                                 size_t hapStride, const uint8_t *haploidRef) {
+    if (!e || !e->initialized) return fail(EIMP_ERR_STATE, "engine not initialized");
+    CHK(set_device(e));
+    if (cM) {
+        CHK(e->cM.ensure(e->M * sizeof(float), "cM"));
+        HIPCHK(hipMemcpy(e->cM.p, cM, e->M * sizeof(float), hipMemcpyHostToDevice));
+        e->pd.cM = e->cM.as<float>();
+        e->have_cM = true;
+    }
+    if (hapData) {
+        if (hapStride < unit_words(e->M) || hapStride % 8)
+            return fail(EIMP_ERR_INVALID, "hapStride must be >= roundToMultiple(M,512)/64 and a multiple of 8");
+        CHK(e->hap.ensure(e->Nad * hapStride * 8, "hap-major panel"));
+        HIPCHK(hipMemcpy(e->hap.p, hapData, e->Nad * hapStride * 8, hipMemcpyHostToDevice));
+        e->hap_stride = hapStride;
+        e->pd.hap = e->hap.as<uint64_t>();
+        e->pd.hap_stride = hapStride;
+        e->have_hap = true;
+    }
+    if (haploidRef) {
+        size_t ns = e->Nad / 2;
+        CHK(e->haploid.ensure(ns ? ns : 1, "haploid flags"));
+        HIPCHK(hipMemcpy(e->haploid.p, haploidRef, ns, hipMemcpyHostToDevice));
+        e->pd.haploid = e->haploid.as<uint8_t>();
+    }
+    return EIMP_OK;
+}
+
+int eimp_engine_run_batch(eimp_engine *e, size_t n, const eimp_target *targets, float cMmaxSplit,
+                          uint32_t flags, eimp_target_result *results) {
+    if (!e || !e->initialized) return fail(EIMP_ERR_STATE, "engine not initialized");
+    if (n == 0) return EIMP_OK;
+    if (!targets || !results) return fail(EIMP_ERR_INVALID, "null targets/results");
+    if (!e->have_cM) return fail(EIMP_ERR_STATE, "genetic positions not set (eimp_engine_set_panel_extras)");
+    if (n > (1u << 24)) return fail(EIMP_ERR_UNSUPPORTED, "too many targets in one batch");
+    CHK(set_device(e));
+    const uint32_t nt = (uint32_t)n;
+    const uint64_t Nw = unit_words(e->Nad);
+    // ---- sizes and validation
+    std::vector<uint64_t> gt_off(nt), ww_off(nt), ss_off(nt), nerr_off(nt), bh_off(nt), bhf_off(nt),
+        mono_off(nt);
+    uint64_t gt_words = 0, w_words = 0, ss_elems = 0, nerr_elems = 0, bh_elems = 0, mono_words = 0;
+    bool need_hap = false, need_pmono = false;
+    std::vector<uint32_t> sel_list, mono_list;
+    uint32_t max_ww = 0;
+    e->h_tg.assign(nt, TgtDev{});
+    for (uint32_t t = 0; t < nt; t++) {
+        const eimp_target &x = targets[t];
+        if (!x.gt) return fail(EIMP_ERR_INVALID, "target " + std::to_string(t) + ": null genotypes");
+        if (x.W < 1 || x.s0 + x.W > e->M)
+            return fail(EIMP_ERR_INVALID, "target " + std::to_string(t) + ": window outside the panel");
+        if (x.K < 1) return fail(EIMP_ERR_INVALID, "target " + std::to_string(t) + ": K < 1");
+        if (x.ignoreSample >= (int64_t)(e->Nad / 2))
+            return fail(EIMP_ERR_INVALID, "target " + std::to_string(t) + ": ignoreSample out of range");
+        TgtDev &T = e->h_tg[t];
+        // Ncorr is the engine's (the caller accounts for the skipped sample,
+        // src/PBWTPhaser.cpp:629); localK = min(Ncorr, K) (:683)
+        const uint64_t ncorr = e->Ncorr;
+        T.K = (uint32_t)std::min<uint64_t>(x.K, ncorr);
+        T.W = (uint32_t)x.W;
+        T.s0 = x.s0;
+        T.ignore = (int32_t)x.ignoreSample;
+        T.cMmaxSplit = cMmaxSplit;
+        const bool select = T.K < ncorr;
+        const bool ident = !select && e->Nad == e->Ncorr && x.ignoreSample < 0 && T.K == e->Nad;
+        T.flags = (select ? TF_SELECT : 0u) | (ident ? TF_IDENTITY : 0u);
+        gt_off[t] = gt_words;
+        gt_words += 2 * unit_words(x.W);
+        ww_off[t] = w_words;
+        w_words += unit_words(x.W);
+        ss_off[t] = ss_elems;
+        ss_elems += round_up(x.W + 1, 64);
+        nerr_off[t] = nerr_elems;
+        if (select) {
+            nerr_elems += round_up(e->Nad, 64);
+            sel_list.push_back(t);
+            need_hap = true;
+        }
+        bh_off[t] = bh_elems;
+        bh_elems += round_up(T.K, 64);
+        bhf_off[t] = (uint64_t)t * Nw;
+        mono_off[t] = mono_words;
+        if (ident) {
+            need_pmono = true;
+        } else {
+            mono_words += unit_words(x.W);
+            mono_list.push_back(t);
+            need_hap = true;
+            max_ww = std::max<uint32_t>(max_ww, (uint32_t)((x.W + 63) / 64));
+        }
+    }
+    if (need_hap) CHK(ensure_hap(e));
+    if (need_pmono) CHK(ensure_pmono(e));
+    // ---- pools
+    CHK(e->d_tg.ensure(nt * sizeof(TgtDev), "target table"));
+    CHK(e->d_gt.ensure(gt_words * 8, "genotypes"));
+    CHK(e->d_nerr.ensure(std::max<uint64_t>(nerr_elems, 1) * 4, "distances"));
+    CHK(e->d_bh.ensure(bh_elems * 4, "best haps"));
+    CHK(e->d_bhflags.ensure((uint64_t)nt * Nw * 8, "best-hap flags"));
+    CHK(e->d_mono.ensure(std::max<uint64_t>(mono_words, 1) * 16, "monos"));
+    CHK(e->d_splitsites.ensure(ss_elems * 4, "split sites"));
+    CHK(e->d_splits.ensure(w_words * 8, "split bitmap"));
+    CHK(e->d_cMdiffs.ensure(ss_elems * 4, "cM diffs"));
+    CHK(e->d_stats.ensure((uint64_t)nt * 16, "stats"));
+    CHK(e->d_tlist.ensure((uint64_t)(sel_list.size() + mono_list.size() + 2) * 4, "target lists"));
+    CHK(e->h_stage.ensure(gt_words * 8));
+    // ---- genotypes to the device (one copy from pinned staging)
+    for (uint32_t t = 0; t < nt; t++) {
+        const uint64_t w = 2 * unit_words(targets[t].W);
+        std::memcpy(e->h_stage.as<uint64_t>() + gt_off[t], targets[t].gt, w * 8);
+    }
+    HIPCHK(hipMemcpyAsync(e->d_gt.p, e->h_stage.p, gt_words * 8, hipMemcpyHostToDevice, e->stream));
+    uint64_t *mono_base = e->d_mono.as<uint64_t>();
+    const uint64_t mono_total = std::max<uint64_t>(mono_words, 1);
+    for (uint32_t t = 0; t < nt; t++) {
+        TgtDev &T = e->h_tg[t];
+        T.gt = e->d_gt.as<uint64_t>() + gt_off[t];
+        T.nerr = e->d_nerr.as<uint32_t>() + nerr_off[t];
+        T.bh = e->d_bh.as<uint32_t>() + bh_off[t];
+        T.bhflags = e->d_bhflags.as<uint64_t>() + bhf_off[t];
+        if (T.flags & TF_IDENTITY) {
+            T.mand = e->pand.as<uint64_t>();
+            T.mor = e->por.as<uint64_t>();
+            T.mand_w = nullptr;
+            T.mor_w = nullptr;
+            T.mono_off = T.s0;
+        } else {
+            T.mand_w = mono_base + mono_off[t];
+            T.mor_w = mono_base + mono_total + mono_off[t];
+            T.mand = T.mand_w;
+            T.mor = T.mor_w;
+            T.mono_off = 0;
+        }
+        T.splitsites = e->d_splitsites.as<uint32_t>() + ss_off[t];
+        T.splits = e->d_splits.as<uint64_t>() + ww_off[t];
+        T.cMdiffs = e->d_cMdiffs.as<float>() + ss_off[t];
+        T.stats = e->d_stats.as<uint32_t>() + 4 * t;
+    }
+    std::vector<uint32_t> lists(sel_list);
+    lists.insert(lists.end(), mono_list.begin(), mono_list.end());
+    if (!lists.empty())
+        HIPCHK(hipMemcpyAsync(e->d_tlist.p, lists.data(), lists.size() * 4, hipMemcpyHostToDevice,
+                              e->stream));
+    HIPCHK(hipMemcpyAsync(e->d_tg.p, e->h_tg.data(), nt * sizeof(TgtDev), hipMemcpyHostToDevice,
+                          e->stream));
+    const uint32_t *d_sel = e->d_tlist.as<uint32_t>();
+    const uint32_t *d_mono_list = d_sel + sel_list.size();
+    // ---- phase A
+    int ti = e->tbegin(EIMP_KCLASS_DISTANCE);
+    HIPCHK(launch_distance(e->pd, e->d_tg.as<TgtDev>(), d_sel, (uint32_t)sel_list.size(), e->stream));
+    e->tend(ti);
+    ti = e->tbegin(EIMP_KCLASS_SELECT);
+    HIPCHK(launch_select(e->pd, e->d_tg.as<TgtDev>(), nt, e->stream));
+    e->tend(ti);
+    ti = e->tbegin(EIMP_KCLASS_MONOS);
+    HIPCHK(launch_monos(e->pd, e->d_tg.as<TgtDev>(), d_mono_list, (uint32_t)mono_list.size(), max_ww,
+                        e->stream));
+    e->tend(ti);
+    ti = e->tbegin(EIMP_KCLASS_SPLITS);
+    HIPCHK(launch_splits(e->pd, e->d_tg.as<TgtDev>(), nt, e->stream));
+    e->tend(ti);
+    e->h_stats.resize(4 * (size_t)nt);
+    HIPCHK(hipMemcpyAsync(e->h_stats.data(), e->d_stats.p, (size_t)nt * 16, hipMemcpyDeviceToHost,
+                          e->stream));
+    CHK(e->sync());
+    for (uint32_t t = 0; t < nt; t++) e->h_tg[t].S = e->h_stats[4 * t];
+    // ---- phase B
+    const bool digest = (flags & EIMP_BATCH_DIGEST) != 0;
+    CHK(run_build(e, nt, digest, (flags & EIMP_BATCH_COMPACT) != 0));
+    std::vector<uint64_t> dig(digest ? 2 * (size_t)nt : 0);
+    if (digest)
+        HIPCHK(hipMemcpyAsync(dig.data(), e->d_digest.p, (size_t)nt * 16, hipMemcpyDeviceToHost,
+                              e->stream));
+    CHK(e->sync());
+    for (uint32_t t = 0; t < nt; t++) {
+        const TgtDev &T = e->h_tg[t];
+        results[t].S = T.S;
+        results[t].Mp = T.Mp;
+        results[t].ncalls = e->h_stats[4 * t + 1];
+        results[t].K = T.K;
+        results[t].digest_pbwt = digest ? dig[2 * t] : 0;
+        results[t].digest_count0 = digest ? dig[2 * t + 1] : 0;
+    }
+    e->last_n = nt;
+    return EIMP_OK;
+}
+
+int eimp_engine_run_cref_pbwt(eimp_engine *e, size_t n, const uint64_t *const *tgtData,
+                              const uint64_t *const *splitdata, const uint64_t *const *besthapdata,
+                              const uint32_t *crefTOffsets, uint32_t *const *cPBWT,
+                              int32_t *const *count0s) {
+    if (!e || !e->initialized) return fail(EIMP_ERR_STATE, "engine not initialized");
+    if (n == 0) return EIMP_OK;
+    if (!tgtData || !splitdata || !besthapdata || !crefTOffsets || !cPBWT || !count0s)
+        return fail(EIMP_ERR_INVALID, "null argument");
+    CHK(set_device(e));
+    const uint32_t nt = (uint32_t)n;
+    const uint64_t M = e->M;
+    const uint64_t gtw = 2 * unit_words(M);                 // our per-target GT stride (words)
+    const uint64_t gt_read = round_up(2 * M, 512) / 64;     // words the reference copies (tgtGTSize)
+    const uint64_t spw = unit_words(M);                     // split bitmap words
+    const uint64_t bhw = e->refTCap / 8;                    // best-hap bitmap words (refTCapacity)
+    for (uint32_t t = 0; t < nt; t++) {
+        if (crefTOffsets[t + 1] < crefTOffsets[t] || ((crefTOffsets[t + 1] - crefTOffsets[t]) % 2) == 0)
+            return fail(EIMP_ERR_INVALID, "crefTOffsets must grow by odd Mp = 2*S+1");
+    }
+    e->h_tg.assign(nt, TgtDev{});
+    const uint64_t stage_words = (uint64_t)nt * (gtw + spw + bhw);
+    CHK(e->h_stage.ensure(stage_words * 8));
+    uint64_t *st = e->h_stage.as<uint64_t>();
+    std::memset(st, 0, stage_words * 8);
+    for (uint32_t t = 0; t < nt; t++) {
+        std::memcpy(st + (uint64_t)t * gtw, tgtData[t], std::min(gt_read, gtw) * 8);
+        std::memcpy(st + (uint64_t)nt * gtw + (uint64_t)t * spw, splitdata[t], spw * 8);
+        std::memcpy(st + (uint64_t)nt * (gtw + spw) + (uint64_t)t * bhw, besthapdata[t], bhw * 8);
+    }
+    CHK(e->d_tg.ensure(nt * sizeof(TgtDev), "target table"));
+    CHK(e->d_gt.ensure(stage_words * 8, "drop-in inputs"));
+    CHK(e->d_bh.ensure((uint64_t)nt * round_up(e->K, 64) * 4, "best haps"));
+    CHK(e->d_splitsites.ensure((uint64_t)nt * round_up(M + 1, 64) * 4, "split sites"));
+    CHK(e->d_stats.ensure((uint64_t)nt * 16, "stats"));
+    HIPCHK(hipMemcpyAsync(e->d_gt.p, st, stage_words * 8, hipMemcpyHostToDevice, e->stream));
+    const bool ident = e->K == e->Nad;
+    for (uint32_t t = 0; t < nt; t++) {
+        TgtDev &T = e->h_tg[t];
+        T.gt = e->d_gt.as<uint64_t>() + (uint64_t)t * gtw;
+        T.s0 = 0;
+        T.W = (uint32_t)M;
+        T.K = (uint32_t)e->K;
+        T.ignore = -1;
+        T.flags = ident ? TF_IDENTITY : 0u;
+        T.splits = e->d_gt.as<uint64_t>() + (uint64_t)nt * gtw + (uint64_t)t * spw;
+        T.bhflags = e->d_gt.as<uint64_t>() + (uint64_t)nt * (gtw + spw) + (uint64_t)t * bhw;
+        T.bh = e->d_bh.as<uint32_t>() + (uint64_t)t * round_up(e->K, 64);
+        T.splitsites = e->d_splitsites.as<uint32_t>() + (uint64_t)t * round_up(M + 1, 64);
+        T.stats = e->d_stats.as<uint32_t>() + 4 * t;
+    }
+    HIPCHK(hipMemcpyAsync(e->d_tg.p, e->h_tg.data(), nt * sizeof(TgtDev), hipMemcpyHostToDevice,
+                          e->stream));
+    HIPCHK(launch_bits_to_list(e->d_tg.as<TgtDev>(), nt, (uint32_t)e->Nad, 0, e->stream));
+    HIPCHK(launch_bits_to_list(e->d_tg.as<TgtDev>(), nt, (uint32_t)e->Nad, 1, e->stream));
+    e->h_stats.resize(4 * (size_t)nt);
+    HIPCHK(hipMemcpyAsync(e->h_stats.data(), e->d_stats.p, (size_t)nt * 16, hipMemcpyDeviceToHost,
+                          e->stream));
+    CHK(e->sync());
+    for (uint32_t t = 0; t < nt; t++) {
+        const uint32_t S = e->h_stats[4 * t], nbh = e->h_stats[4 * t + 3];
+        if (nbh != e->K)
+            return fail(EIMP_ERR_INVALID, "target " + std::to_string(t) + ": best-hap bitmap has " +
+                                              std::to_string(nbh) + " bits set, K = " + std::to_string(e->K));
+        if (2ull * S + 1 != crefTOffsets[t + 1] - crefTOffsets[t])
+            return fail(EIMP_ERR_INVALID, "target " + std::to_string(t) +
+                                              ": crefTOffsets disagree with the split bitmap");
+        e->h_tg[t].S = S;
+    }
+    CHK(run_build(e, nt, false, false));
+    // results to the caller's host buffers, as src/GPUEngine.cu:182-211
+    for (uint32_t t = 0; t < nt; t++) {
+        const TgtDev &T = e->h_tg[t];
+        const int ti = e->tbegin(EIMP_KCLASS_COPY);
+        HIPCHK(hipMemcpyAsync(cPBWT[t], T.out, 2ull * T.Mp * T.RS * 4, hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(hipMemcpyAsync(count0s[t], T.count0, (size_t)T.Mp * 4, hipMemcpyDeviceToHost, e->stream));
+        e->tend(ti);
+    }
+    CHK(e->sync());
+    e->last_n = nt;
+    return EIMP_OK;
+}
+
+int eimp_engine_copy_target(eimp_engine *e, size_t t, uint32_t *besthaps, uint64_t *bhflags,
+                            uint32_t *splitsites, uint64_t *splits, float *cMdiffs, int32_t *count0,
+                            uint32_t *pbwt, uint64_t *cref_rows) {
+    if (!e || !e->initialized) return fail(EIMP_ERR_STATE, "engine not initialized");
+    if (t >= e->last_n || t >= e->h_tg.size()) return fail(EIMP_ERR_INVALID, "target index out of range");
+    CHK(set_device(e));
+    const TgtDev &T = e->h_tg[t];
+    if (besthaps && T.bh) HIPCHK(hipMemcpy(besthaps, T.bh, (size_t)T.K * 4, hipMemcpyDeviceToHost));
+    if (bhflags && T.bhflags)
+        HIPCHK(hipMemcpy(bhflags, T.bhflags, unit_words(e->Nad) * 8, hipMemcpyDeviceToHost));
+    if (splitsites && T.S) HIPCHK(hipMemcpy(splitsites, T.splitsites, (size_t)T.S * 4, hipMemcpyDeviceToHost));
+    if (splits && T.splits) HIPCHK(hipMemcpy(splits, T.splits, unit_words(T.W) * 8, hipMemcpyDeviceToHost));
+    if (cMdiffs && T.S && T.cMdiffs)
+        HIPCHK(hipMemcpy(cMdiffs, T.cMdiffs, ((size_t)T.S + 1) * 4, hipMemcpyDeviceToHost));
+    if (count0) HIPCHK(hipMemcpy(count0, T.count0, (size_t)T.Mp * 4, hipMemcpyDeviceToHost));
+    if (pbwt) HIPCHK(hipMemcpy(pbwt, T.out, 2ull * T.Mp * T.RS * 4, hipMemcpyDeviceToHost));
+    if (cref_rows) {
+        const uint32_t rw = (uint32_t)unit_words(T.K);
+        CHK(e->d_tmp.ensure((size_t)T.Mp * rw * 8, "cref rows"));
+        HIPCHK(launch_cref_rows(e->d_tg.as<TgtDev>(), (uint32_t)t, T.Mp, e->d_tmp.as<uint64_t>(), rw,
+                                e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+        HIPCHK(hipMemcpy(cref_rows, e->d_tmp.p, (size_t)T.Mp * rw * 8, hipMemcpyDeviceToHost));
+    }
+    return EIMP_OK;
+}
+
+int eimp_engine_set_timing(eimp_engine *e, int enable) {
+    if (!e) return fail(EIMP_ERR_INVALID, "null engine");
+    e->timing = enable != 0;
+    return EIMP_OK;
+}
+
+int eimp_engine_get_timing(eimp_engine *e, double *ms, uint64_t *launches) {
+    if (!e) return fail(EIMP_ERR_INVALID, "null engine");
+    for (int i = 0; i < EIMP_NKCLASS; i++) {
+        if (ms) ms[i] = e->ms[i];
+        if (launches) launches[i] = e->launches[i];
+    }
+    return EIMP_OK;
+}
+
+int eimp_engine_reset_timing(eimp_engine *e) {
+    if (!e) return fail(EIMP_ERR_INVALID, "null engine");
+    for (int i = 0; i < EIMP_NKCLASS; i++) {
+        e->ms[i] = 0;
+        e->launches[i] = 0;
+    }
+    return EIMP_OK;
+}
+
+int eimp_engine_set_pbwt_variant(eimp_engine *e, int variant) {
+    if (!e) return fail(EIMP_ERR_INVALID, "null engine");
+    if (variant >= pbwt_variant_count()) return fail(EIMP_ERR_INVALID, "variant out of range");
+    e->pbwt_variant = variant < 0 ? -1 : variant;
+    return EIMP_OK;
+}
+
+int eimp_pbwt_variant_count(void) { return pbwt_variant_count(); }
+
+int eimp_pbwt_variant_info(int v, int *threads, int *cpw, int *max_k) {
+    if (v < 0 || v >= pbwt_variant_count()) return fail(EIMP_ERR_INVALID, "variant out of range");
+    if (threads) *threads = pbwt_variant_threads(v);
+    if (cpw) *cpw = pbwt_variant_cpw(v);
+    if (max_k) *max_k = (int)pbwt_variant_maxk(v);
+    return EIMP_OK;
+}
+
+}  // extern "C"

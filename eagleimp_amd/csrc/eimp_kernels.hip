@@ -1,0 +1,821 @@
+// eimp_kernels.hip -- gfx950 kernels of the phasing pre-stage.
+//
+// Everything here is integer/bitwise work; wave64 ballots and LDS do the
+// per-site and per-step partitions, HBM is streamed with coalesced loads.
+//
+//   k_site_to_hap    site-major -> hap-major panel (64x64 bit tiles via LDS)
+//   k_panel_mono     per-site "all ones" / "any one" bits of the whole panel
+//   k_distance       nerr(h) = sum popcount((is0 & h) | (is2 & ~h))
+//                    (GenotypeVector::compare_withPreInit, src/Datatypes.cpp:107-116)
+//   k_select         exact K smallest (nerr, h), ascending h
+//                    (Target::findBestHaps, src/Target.cpp:779-875)
+//   k_monos          AND / OR of the chosen rows per window word
+//                    (Target::findMonos, src/Target.cpp:878-897)
+//   k_splits         split / call sites, one wave per target
+//                    (Target::findCallLocs, src/Target.cpp:900-950)
+//   k_bits_to_list   bitmap -> ascending index list (drop-in inputs)
+//   k_cref           condensed reference in 16-row transposed blocks
+//                    (Target::condensedReferenceT, src/Target.cpp:953-1059)
+//   k_pbwt           compressed forward / backward PBWT
+//                    (PBWT::buildComprPBWT, src/PBWT.cpp:207-285)
+#include "eimp_internal.h"
+
+namespace eimp {
+
+// ---------------------------------------------------------------------------
+// wave helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    uint32_t l = lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor(v, o));
+    return v;
+}
+// inclusive scan over the wave
+__device__ __forceinline__ uint32_t wave_scan_incl(uint32_t v) {
+    uint32_t l = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t u = __shfl_up(v, o);
+        if (l >= (uint32_t)o) v += u;
+    }
+    return v;
+}
+
+// bits [bitoff, bitoff+64) of a bit row with `words` readable words
+__device__ __forceinline__ uint64_t funnel(const uint64_t *__restrict__ row, uint64_t bitoff,
+                                           uint64_t words) {
+    uint64_t w = bitoff >> 6;
+    uint32_t sh = (uint32_t)(bitoff & 63);
+    uint64_t lo = w < words ? row[w] : 0ull;
+    if (sh == 0) return lo;
+    uint64_t hi = (w + 1) < words ? row[w + 1] : 0ull;
+    return (lo >> sh) | (hi << (64 - sh));
+}
+
+__device__ __forceinline__ bool excluded_hap(const PanelDev &P, int32_t ignore, uint32_t h) {
+    uint32_t nr = h >> 1;
+    if ((int32_t)nr == ignore) return true;
+    return (h & 1u) && P.haploid && P.haploid[nr];
+}
+
+// ---------------------------------------------------------------------------
+// panel set-up
+// ---------------------------------------------------------------------------
+// grid (ceil(hap_stride/4), ceil(N_ad/64)), block 256: one 64-hap x 256-site tile
+__global__ __launch_bounds__(256) void k_site_to_hap(PanelDev P, uint64_t *__restrict__ hap) {
+    __shared__ uint64_t tile[256];
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, l = lane_id();
+    const uint64_t hb = blockIdx.y;
+    const uint64_t sw0 = (uint64_t)blockIdx.x * 4;
+    const uint64_t m = sw0 * 64 + tid;
+    tile[tid] = (m < P.M && hb < P.site_stride) ? P.site[m * P.site_stride + hb] : 0ull;
+    __syncthreads();
+    uint64_t out = 0;
+#pragma unroll 8
+    for (int j = 0; j < 64; j++) out |= ((tile[wave * 64 + j] >> l) & 1ull) << j;
+    const uint64_t h = hb * 64 + l, sw = sw0 + wave;
+    if (h < P.N_ad && sw < P.hap_stride) hap[h * P.hap_stride + sw] = out;
+}
+
+// grid ceil(M/4), block 256: one wave per site
+__global__ __launch_bounds__(256) void k_panel_mono(PanelDev P, uint64_t *__restrict__ pand,
+                                                   uint64_t *__restrict__ por) {
+    const uint32_t wave = threadIdx.x >> 6, l = lane_id();
+    const uint64_t m = (uint64_t)blockIdx.x * 4 + wave;
+    if (m >= P.M) return;
+    const uint64_t nw = (P.N_ad + 63) / 64;
+    uint32_t c = 0;
+    for (uint64_t w = l; w < nw; w += 64) c += __popcll(P.site[m * P.site_stride + w]);
+    c = wave_sum(c);
+    if (l == 0) {
+        unsigned long long bit = 1ull << (m & 63);
+        if (c == P.N_ad) atomicOr((unsigned long long *)&pand[m >> 6], bit);
+        if (c > 0) atomicOr((unsigned long long *)&por[m >> 6], bit);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// a3: distances.  grid (ceil(N_ad/16), nsel), block 256, a wave per haplotype
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_distance(PanelDev P, const TgtDev *__restrict__ tg,
+                                                 const uint32_t *__restrict__ tlist) {
+    const TgtDev &T = tg[tlist[blockIdx.y]];
+    const uint32_t wave = threadIdx.x >> 6, l = lane_id();
+    const uint64_t Ww = (T.W + 63) / 64;
+    const uint64_t *__restrict__ gt = T.gt;
+#pragma unroll 1
+    for (uint32_t hh = 0; hh < 4; hh++) {
+        const uint32_t h = blockIdx.x * 16 + wave * 4 + hh;
+        if (h >= P.N_ad) break;
+        if (excluded_hap(P, T.ignore, h)) {
+            if (l == 0) T.nerr[h] = kExcluded;
+            continue;
+        }
+        const uint64_t *__restrict__ row = P.hap + (uint64_t)h * P.hap_stride;
+        uint32_t e = 0;
+        for (uint64_t i = l; i < Ww; i += 64) {
+            uint64_t hv = funnel(row, T.s0 + 64 * i, P.hap_stride);
+            e += __popcll((gt[2 * i] & hv) | (gt[2 * i + 1] & ~hv));
+        }
+        e = wave_sum(e);
+        if (l == 0) T.nerr[h] = e;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// a4: exact top-K selection / take-all.  grid n, block 1024
+// ---------------------------------------------------------------------------
+constexpr int SEL_NT = 1024;
+constexpr int SEL_NW = SEL_NT / 64;
+
+// block-wide exclusive scan of one value per thread; also returns the total
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *s_w, uint32_t &total) {
+    const uint32_t wave = threadIdx.x >> 6, l = lane_id();
+    uint32_t inc = wave_scan_incl(v);
+    if (l == 63) s_w[wave] = inc;
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < SEL_NW; w++) {
+        uint32_t x = s_w[w];
+        tot += x;
+        if (w < (int)wave) before += x;
+    }
+    __syncthreads();
+    total = tot;
+    return before + inc - v;
+}
+
+__global__ __launch_bounds__(SEL_NT) void k_select(PanelDev P, const TgtDev *__restrict__ tg) {
+    const TgtDev &T = tg[blockIdx.x];
+    __shared__ uint32_t hist[1024];
+    __shared__ uint32_t s_w[SEL_NW];
+    __shared__ uint32_t s_bin, s_before;
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, l = lane_id();
+    const uint32_t N = P.N_ad;
+    const bool select = (T.flags & TF_SELECT) != 0;
+    uint32_t thr = 0, need = 0;
+    if (select) {
+        const uint32_t *__restrict__ nerr = T.nerr;
+        uint32_t mx = 0;
+        for (uint32_t h = tid; h < N; h += SEL_NT) {
+            uint32_t v = nerr[h];
+            if (v != kExcluded) mx = max(mx, v);
+        }
+        mx = wave_max(mx);
+        if (l == 0) s_w[wave] = mx;
+        __syncthreads();
+        mx = 0;
+        for (int w = 0; w < SEL_NW; w++) mx = max(mx, s_w[w]);
+        __syncthreads();
+        int hi = mx ? 32 - __clz(mx) : 0;
+        uint32_t k = T.K;  // rank (1-based) of the threshold value
+        uint64_t prefix = 0;
+        while (hi > 0) {
+            const int lo = hi > 10 ? hi - 10 : 0;
+            const uint32_t nb = 1u << (hi - lo);
+            for (uint32_t b = tid; b < nb; b += SEL_NT) hist[b] = 0;
+            __syncthreads();
+            for (uint32_t h = tid; h < N; h += SEL_NT) {
+                uint32_t v = nerr[h];
+                if (v != kExcluded && ((uint64_t)v >> hi) == (prefix >> hi))
+                    atomicAdd(&hist[(v >> lo) & (nb - 1)], 1u);
+            }
+            __syncthreads();
+            uint32_t c = tid < nb ? hist[tid] : 0u, total;
+            uint32_t ex = block_excl_scan(c, s_w, total);
+            if (tid < nb && ex < k && k <= ex + c) {
+                s_bin = tid;
+                s_before = ex;
+            }
+            __syncthreads();
+            k -= s_before;
+            prefix |= (uint64_t)s_bin << lo;
+            hi = lo;
+            __syncthreads();
+        }
+        thr = (uint32_t)prefix;
+        need = k;
+    }
+    // compaction in index order: ties at the threshold value go to the lowest indices
+    const uint32_t nwords = ((N + 511) / 512) * 8;
+    uint32_t ties_before = 0, sel_before = 0;
+    for (uint32_t base = 0; base < nwords * 64; base += SEL_NT) {
+        const uint32_t h = base + tid;
+        bool sel, tie = false;
+        if (h >= N) {
+            sel = false;
+        } else if (select) {
+            uint32_t v = T.nerr[h];
+            tie = (v == thr);
+            sel = v < thr;
+        } else {
+            sel = !excluded_hap(P, T.ignore, h);
+        }
+        if (select) {
+            uint32_t tcount;
+            uint32_t trank = ties_before + block_excl_scan(tie ? 1u : 0u, s_w, tcount);
+            if (tie && trank < need) sel = true;
+            ties_before += tcount;
+        }
+        uint64_t sm = __ballot(sel);
+        const uint32_t word = base / 64 + wave;
+        if (l == 0 && word < nwords) T.bhflags[word] = sm;
+        uint32_t scount;
+        uint32_t pos = sel_before + block_excl_scan(sel ? 1u : 0u, s_w, scount);
+        if (sel && pos < T.K) T.bh[pos] = h;  // exactly K when Ncorr is consistent
+        sel_before += scount;
+    }
+    if (tid == 0) T.stats[3] = sel_before;
+}
+
+// ---------------------------------------------------------------------------
+// a5: monos for targets that select a subset.  grid (ceil(Ww/64), n), block
+// 1024: 16 waves split the K chosen rows, lanes cover 64 window words.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_monos(PanelDev P, const TgtDev *__restrict__ tg,
+                                               const uint32_t *__restrict__ tlist) {
+    const TgtDev &T = tg[tlist[blockIdx.y]];
+    __shared__ uint64_t s_and[16][64], s_or[16][64];
+    const uint32_t wave = threadIdx.x >> 6, l = lane_id();
+    const uint64_t Ww = (T.W + 63) / 64;
+    const uint64_t i = (uint64_t)blockIdx.x * 64 + l;
+    const bool valid = i < Ww;
+    uint64_t a = ~0ull, o = 0ull;
+    const uint64_t bitoff = T.s0 + 64 * i;
+    if (valid) {
+        uint32_t k = wave;
+        for (; k + 48 < T.K; k += 64) {
+            uint64_t v0 = funnel(P.hap + (uint64_t)T.bh[k] * P.hap_stride, bitoff, P.hap_stride);
+            uint64_t v1 = funnel(P.hap + (uint64_t)T.bh[k + 16] * P.hap_stride, bitoff, P.hap_stride);
+            uint64_t v2 = funnel(P.hap + (uint64_t)T.bh[k + 32] * P.hap_stride, bitoff, P.hap_stride);
+            uint64_t v3 = funnel(P.hap + (uint64_t)T.bh[k + 48] * P.hap_stride, bitoff, P.hap_stride);
+            a &= v0 & v1 & v2 & v3;
+            o |= v0 | v1 | v2 | v3;
+        }
+        for (; k < T.K; k += 16) {
+            uint64_t v = funnel(P.hap + (uint64_t)T.bh[k] * P.hap_stride, bitoff, P.hap_stride);
+            a &= v;
+            o |= v;
+        }
+    }
+    s_and[wave][l] = a;
+    s_or[wave][l] = o;
+    __syncthreads();
+    if (wave == 0 && valid) {
+        for (int w = 1; w < 16; w++) {
+            a &= s_and[w][l];
+            o |= s_or[w][l];
+        }
+        T.mand_w[i] = a;
+        T.mor_w[i] = o;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// a6: split and call sites.  grid n, block 64 (one wave per target).
+// Sequential over 64-site chunks; within a chunk het candidates are always
+// splits and hom candidates are resolved lowest-first against the cM of the
+// previous split (see the comment in the loop).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_splits(PanelDev P, const TgtDev *__restrict__ tg) {
+    const TgtDev &T = tg[blockIdx.x];
+    const uint32_t l = lane_id();
+    const uint32_t W = T.W;
+    const uint32_t Ww = (W + 63) / 64;
+    const float *__restrict__ cMw = P.cM + T.s0;
+    const float thr = T.cMmaxSplit;
+    const uint64_t lt = lanemask_lt();
+    const uint64_t mono_words = T.mand == T.mand_w ? (uint64_t)((W + 511) / 512) * 8
+                                                   : (uint64_t)((P.M + 511) / 512) * 8;
+    float Lprev = cMw[0];
+    bool started = false;
+    uint32_t S = 0, ncalls = 0;
+    int64_t lastcall = -1;      // index (in splitsites) of the last het split
+    float lastcall_cm = 0.f;
+    uint32_t lastcall_word = 0;
+    uint64_t lastcall_mask = 0;
+    for (uint32_t c = 0; c < Ww; c++) {
+        const uint32_t m = c * 64 + l;
+        const bool valid = m < W;
+        const uint64_t g0 = T.gt[2 * c], g2 = T.gt[2 * c + 1];
+        const bool is0 = (g0 >> l) & 1ull, is2 = (g2 >> l) & 1ull;
+        const bool het = valid && !is0 && !is2;
+        const bool miss = is0 && is2;
+        const uint64_t hetm = __ballot(het);
+        bool elig = true;
+        if (!started) {
+            if (hetm == 0ull) continue;  // the phasing starts at the first het (:907-913)
+            started = true;
+            elig = l >= (uint32_t)(__ffsll((unsigned long long)hetm) - 1);
+        }
+        const uint64_t ma = funnel(T.mand, T.mono_off + 64ull * c, mono_words);
+        const uint64_t mo = funnel(T.mor, T.mono_off + 64ull * c, mono_words);
+        const bool mono = ((ma >> l) & 1ull) || !((mo >> l) & 1ull);
+        const bool cand = valid && elig && !mono && !miss;
+        const uint64_t hetc = __ballot(cand && het);
+        const uint64_t homc = __ballot(cand && !het);
+        const float cm = valid ? cMw[m] : 0.f;
+        uint64_t splitm = hetc, pending = homc;
+        // Resolve hom candidates.  All splits fixed so far are true splits and
+        // L (cM of the previous split) only grows as splits are added, so a
+        // pending candidate failing now fails for good, and the lowest one
+        // passing now has exactly the true L: accept it and repeat.
+        while (pending) {
+            const uint64_t below = splitm & lt;
+            const int src = below ? 63 - __clzll((long long)below) : (int)l;
+            const float Ls = __shfl(cm, src);
+            const float L = below ? Ls : Lprev;
+            const float d = cm - L;
+            const bool pass = ((pending >> l) & 1ull) && (d > thr);
+            const uint64_t passm = __ballot(pass);
+            pending &= passm;
+            if (!passm) break;
+            const uint64_t p = passm & (~passm + 1);  // lowest passing lane
+            splitm |= p;
+            pending &= ~p;
+        }
+        if (splitm) {
+            // emit split sites and their cM differences (src/Target.cpp:943-947)
+            const uint64_t below = splitm & lt;
+            const int src = below ? 63 - __clzll((long long)below) : (int)l;
+            const float Ls = __shfl(cm, src);
+            const float L = below ? Ls : Lprev;
+            if ((splitm >> l) & 1ull) {
+                const uint32_t idx = S + (uint32_t)__popcll(below);
+                T.splitsites[idx] = m;
+                T.cMdiffs[idx] = cm - L;
+            }
+            if (hetc) {
+                const int top = 63 - __clzll((long long)hetc);
+                const uint64_t upto = top == 63 ? ~0ull : ((1ull << (top + 1)) - 1);
+                lastcall = (int64_t)S + __popcll(splitm & upto) - 1;
+                lastcall_cm = __shfl(cm, top);
+                lastcall_word = c;
+                lastcall_mask = splitm & upto;
+                ncalls += (uint32_t)__popcll(hetc);
+            }
+            S += (uint32_t)__popcll(splitm);
+            Lprev = __shfl(cm, 63 - __clzll((long long)splitm));
+            if (l == 0) T.splits[c] = splitm;
+        } else if (l == 0) {
+            T.splits[c] = 0ull;
+        }
+    }
+    // no split, or no call: S = 0 (:927-935); else truncate after the last call (:939-940)
+    uint32_t Sfin = (S == 0 || ncalls == 0) ? 0u : (uint32_t)(lastcall + 1);
+    const uint32_t nwords = ((W + 511) / 512) * 8;
+    const uint32_t keep_to = Sfin ? lastcall_word : 0u;
+    // words below keep_to hold the loop's values (k_splits_clear zeroed the
+    // words the loop skipped before the first het)
+    for (uint32_t w = l; w < nwords; w += 64) {
+        if (Sfin == 0 || w > keep_to) T.splits[w] = 0ull;
+        else if (w == keep_to) T.splits[w] = lastcall_mask;
+    }
+    if (l == 0) {
+        if (Sfin) T.cMdiffs[Sfin] = cMw[W - 1] - lastcall_cm;
+        T.stats[0] = Sfin;
+        T.stats[1] = Sfin ? ncalls : 0u;
+    }
+}
+
+// Words skipped before the first het must be zero in the split bitmap: the
+// loop `continue`s there without writing.  This pass clears them (grid n,
+// block 64) -- it runs before k_splits.
+__global__ __launch_bounds__(64) void k_splits_clear(const TgtDev *__restrict__ tg) {
+    const TgtDev &T = tg[blockIdx.x];
+    const uint32_t nwords = ((T.W + 511) / 512) * 8;
+    for (uint32_t w = lane_id(); w < nwords; w += 64) T.splits[w] = 0ull;
+}
+
+// ---------------------------------------------------------------------------
+// bitmap -> ascending index list (drop-in path: best-haps flags, split
+// bitmap).  grid n, block 1024.  which: 0 best haps, 1 split sites.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(SEL_NT) void k_bits_to_list(const TgtDev *__restrict__ tg,
+                                                        uint32_t nbits_bh, int which) {
+    const TgtDev &T = tg[blockIdx.x];
+    __shared__ uint32_t s_w[SEL_NW];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t *bits = which == 0 ? T.bhflags : T.splits;
+    uint32_t *out = which == 0 ? T.bh : T.splitsites;
+    const uint32_t nbits = which == 0 ? nbits_bh : T.W;
+    const uint32_t cap = which == 0 ? T.K : T.W;  // list capacity; the host rejects a mismatch
+    uint32_t before = 0;
+    for (uint32_t base = 0; base < nbits; base += SEL_NT) {
+        const uint32_t i = base + tid;
+        const bool set = i < nbits && ((bits[i >> 6] >> (i & 63)) & 1ull);
+        uint32_t total;
+        uint32_t pos = before + block_excl_scan(set ? 1u : 0u, s_w, total);
+        if (set && pos < cap) out[pos] = i;
+        before += total;
+    }
+    if (tid == 0 && which == 1) T.stats[0] = before;
+    if (tid == 0 && which == 0) T.stats[3] = before;
+}
+
+// ---------------------------------------------------------------------------
+// a7: condensed reference.  One work item = a run of 16-row blocks of one
+// target.  Rows 2s (OR of the inconsistencies of segment s) and 2s+1 (the
+// haplotypes at split s) are gathered to the K chosen haplotypes; each thread
+// owns EPT haplotypes and accumulates their 16 bits of the block in a
+// register, written as uint16 per haplotype (transposed block).
+// grid nitems, block 512.
+// ---------------------------------------------------------------------------
+constexpr int CREF_NT = 512;
+
+template <int EPT>
+__global__ __launch_bounds__(CREF_NT) void k_cref(PanelDev P, const TgtDev *__restrict__ tg,
+                                                 const CrefItem *__restrict__ items) {
+    extern __shared__ uint64_t sm64[];
+    const CrefItem it = items[blockIdx.x];
+    const TgtDev &T = tg[it.t];
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, l = lane_id();
+    const uint32_t K = T.K, S = T.S, W = T.W, Kpad = T.Kpad, Mp = T.Mp;
+    const uint32_t nw = (P.N_ad + 63) / 64;
+    uint64_t *acc = sm64;            // nw words
+    __shared__ uint32_t cnt[16];
+    const bool ident = (T.flags & TF_IDENTITY) != 0;
+    uint32_t idx[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+        const uint32_t k = tid + CREF_NT * e;
+        idx[e] = k < K ? (ident ? k : T.bh[k]) : 0u;
+    }
+    const uint64_t *__restrict__ site = P.site + T.s0 * P.site_stride;
+    for (uint32_t b = it.b0; b < it.b1; b++) {
+        uint32_t V[EPT];
+#pragma unroll
+        for (int e = 0; e < EPT; e++) V[e] = 0;
+        if (tid < 16) cnt[tid] = 0;
+        for (uint32_t jj = 0; jj < 8; jj++) {
+            const uint32_t s = 8 * b + jj;
+            if (s > S) break;
+            // --- row 2s: inconsistencies of segment s (mergeInconsistencies, :1040-1059)
+            const uint32_t lo = s == 0 ? 0u : T.splitsites[s - 1] + 1;
+            const uint32_t hi = s < S ? T.splitsites[s] : W;
+            for (uint32_t w = tid; w < nw; w += CREF_NT) {
+                uint64_t a = 0;
+                for (uint32_t m = lo; m < hi; m++) {
+                    const uint64_t g0 = T.gt[2 * (m >> 6)], g2 = T.gt[2 * (m >> 6) + 1];
+                    const bool is0 = (g0 >> (m & 63)) & 1ull, is2 = (g2 >> (m & 63)) & 1ull;
+                    if (is0 == is2) continue;  // het or missing: consistent with all
+                    const uint64_t v = site[(uint64_t)m * P.site_stride + w];
+                    a |= is0 ? v : ~v;
+                }
+                acc[w] = a;
+            }
+            __syncthreads();
+            {
+                const uint32_t rb = 2 * jj;
+                uint32_t ones = 0;
+#pragma unroll
+                for (int e = 0; e < EPT; e++) {
+                    const uint32_t k = tid + CREF_NT * e;
+                    const uint32_t bit = k < K ? (uint32_t)((acc[idx[e] >> 6] >> (idx[e] & 63)) & 1ull) : 0u;
+                    V[e] |= bit << rb;
+                    ones += (uint32_t)__popcll(__ballot(bit));
+                }
+                if (l == 0) atomicAdd(&cnt[rb], ones);
+            }
+            __syncthreads();
+            if (s < S) {
+                // --- row 2s+1: haplotypes at split s (:986-1003)
+                const uint64_t *__restrict__ row = site + (uint64_t)T.splitsites[s] * P.site_stride;
+                for (uint32_t w = tid; w < nw; w += CREF_NT) acc[w] = row[w];
+                __syncthreads();
+                const uint32_t rb = 2 * jj + 1;
+                uint32_t ones = 0;
+#pragma unroll
+                for (int e = 0; e < EPT; e++) {
+                    const uint32_t k = tid + CREF_NT * e;
+                    const uint32_t bit = k < K ? (uint32_t)((acc[idx[e] >> 6] >> (idx[e] & 63)) & 1ull) : 0u;
+                    V[e] |= bit << rb;
+                    ones += (uint32_t)__popcll(__ballot(bit));
+                }
+                if (l == 0) atomicAdd(&cnt[rb], ones);
+                __syncthreads();
+            }
+        }
+        uint16_t *__restrict__ dst = T.cref + (uint64_t)b * Kpad;
+#pragma unroll
+        for (int e = 0; e < EPT; e++) {
+            const uint32_t k = tid + CREF_NT * e;
+            if (k < Kpad) dst[k] = (uint16_t)V[e];
+        }
+        __syncthreads();
+        if (tid < 16) {
+            const uint32_t r = 16 * b + tid;
+            if (r < Mp) T.count0[r] = (int32_t)K - (int32_t)cnt[tid];
+        }
+        __syncthreads();
+        (void)wave;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// a9: compressed PBWT.  One workgroup per (target, direction).
+//
+// Position i of the current order is held by wave i/(64*CPW), chunk
+// (i/64)%CPW, lane i%64, as a 32-bit payload (hap index << 16 | the 16 bits
+// of the current cref block for that haplotype).  A step:
+//   pass 1  ballot of the row bit per chunk -> zero counts, wave totals to LDS
+//   pass 2  zeros-before / ones-before bases per chunk, destination of each
+//           payload under the stable partition (zeros first), payloads
+//           scattered into LDS, the ballot words + cumulative zero counts of
+//           the row written to HBM (one lane per chunk)
+//   pass 3  read back the payloads of my positions.
+// At each 16-row block boundary the next block's uint16 values are staged in
+// LDS and gathered by hap index.
+// ---------------------------------------------------------------------------
+template <int NT, int CPW>
+__global__ __launch_bounds__(NT) void k_pbwt(const TgtDev *__restrict__ tg,
+                                            const PbwtItem *__restrict__ items) {
+    constexpr int NW = NT / 64;
+    extern __shared__ uint32_t smem[];
+    const PbwtItem it = items[blockIdx.x];
+    const TgtDev &T = tg[it.t];
+    const uint32_t K = T.K, Mp = T.Mp, Kpad = T.Kpad, RS = T.RS;
+    const uint32_t nch = Kpad / 64;
+    const uint32_t npairs = (K + 31) / 32;
+    const uint32_t KW = 2 * npairs;
+    uint32_t *pay = smem;                                    // Kpad
+    uint16_t *stage = (uint16_t *)(smem + Kpad);             // Kpad
+    uint32_t *tot = smem + Kpad + Kpad / 2;                  // 2*NW
+    const uint32_t wave = threadIdx.x >> 6, ln = lane_id();
+    const bool rev = it.dir != 0;
+    const uint16_t *__restrict__ cref = T.cref;
+    const bool dig = T.digest != nullptr;
+
+    uint32_t p[CPW];
+    {
+        const uint32_t r0 = rev ? Mp - 1 : 0;
+        const uint32_t b0 = r0 >> 4;
+#pragma unroll
+        for (int c = 0; c < CPW; c++) {
+            const uint32_t pos = (wave * CPW + c) * 64 + ln;
+            p[c] = pos < K ? (pos << 16) | (uint32_t)cref[(uint64_t)b0 * Kpad + pos] : 0u;
+        }
+    }
+    uint32_t *__restrict__ outbase = T.out + (rev ? (uint64_t)Mp * RS : 0ull);
+    const uint64_t gbase = rev ? (uint64_t)Mp * KW : 0ull;
+    uint64_t dsum = 0;
+    const uint32_t nslots = RS / 4;  // 4-word slots of a stored row (RS even)
+
+    for (uint32_t step = 0; step < Mp; ++step) {
+        const uint32_t r = rev ? Mp - 1 - step : step;
+        const uint32_t j = r & 15;
+        if (step > 0 && (rev ? j == 15 : j == 0)) {
+            const uint32_t *__restrict__ src = (const uint32_t *)(cref + (uint64_t)(r >> 4) * Kpad);
+            uint32_t *st32 = (uint32_t *)stage;
+            for (uint32_t q = threadIdx.x; q < Kpad / 2; q += NT) st32[q] = src[q];
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < CPW; c++) {
+                const uint32_t pos = (wave * CPW + c) * 64 + ln;
+                if (pos < K) {
+                    const uint32_t id = p[c] >> 16;
+                    p[c] = (id << 16) | (uint32_t)stage[id];
+                }
+            }
+        }
+        // pass 1
+        uint32_t zpre[CPW];
+        uint32_t zw = 0;
+#pragma unroll
+        for (int c = 0; c < CPW; c++) {
+            const uint32_t ch = wave * CPW + c;
+            zpre[c] = zw;
+            if (ch < nch) {
+                const uint32_t pos = ch * 64 + ln;
+                const bool bit = pos < K ? ((p[c] >> j) & 1u) : true;
+                const uint64_t m = __ballot(bit);
+                zw += 64u - (uint32_t)__popcll(m);
+            }
+        }
+        const uint32_t par = step & 1u;
+        if (ln == 0) tot[par * NW + wave] = zw;
+        __syncthreads();
+        uint32_t base = 0, cnt0 = 0;
+#pragma unroll
+        for (int w2 = 0; w2 < NW; w2++) {
+            const uint32_t v = tot[par * NW + w2];
+            cnt0 += v;
+            if (w2 < (int)wave) base += v;
+        }
+        // pass 2
+        uint32_t olo = 0, ohi = 0, ozb = 0;
+#pragma unroll
+        for (int c = 0; c < CPW; c++) {
+            const uint32_t ch = wave * CPW + c;
+            if (ch < nch) {
+                const uint32_t pos = ch * 64 + ln;
+                const bool valid = pos < K;
+                const bool bit = valid ? ((p[c] >> j) & 1u) : true;
+                const uint64_t m = __ballot(bit);
+                const uint32_t t1 = mbcnt(m);
+                const uint32_t zb = base + zpre[c];
+                const uint32_t ob = cnt0 + ch * 64 - zb;
+                const uint32_t dest = bit ? ob + t1 : zb + ln - t1;
+                if (valid) pay[dest] = p[c];
+                if (ln == (uint32_t)c) {
+                    olo = (uint32_t)m;
+                    ohi = (uint32_t)(m >> 32);
+                    ozb = zb;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < CPW; c++) {
+            const uint32_t pos = (wave * CPW + c) * 64 + ln;
+            if (pos < K) p[c] = pay[pos];
+        }
+        // the row: chunk ch holds pairs 2ch and 2ch+1
+        uint32_t *__restrict__ row = outbase + (uint64_t)step * RS;
+        const uint64_t g0 = gbase + (uint64_t)step * KW;
+        {
+            const uint32_t ch = wave * CPW + ln;
+            if (ln < (uint32_t)CPW && ch < nch) {
+                const uint32_t o1 = ozb + 32u - (uint32_t)__popc(olo);
+                const uint32_t o3 = o1 + 32u - (uint32_t)__popc(ohi);
+                const bool second = 2 * ch + 1 < npairs;
+                uint2 *dst = (uint2 *)(row + 4 * ch);
+                dst[0] = make_uint2(olo, o1);
+                if (second) dst[1] = make_uint2(ohi, o3);
+                else if (4 * ch + 2 < RS) dst[1] = make_uint2(0u, 0u);
+                if (dig) {
+                    const uint64_t g = g0 + 4ull * ch;
+                    dsum += (uint64_t)dig_word(g, olo) + (uint64_t)dig_word(g + 1, o1);
+                    if (second) dsum += (uint64_t)dig_word(g + 2, ohi) + (uint64_t)dig_word(g + 3, o3);
+                }
+            }
+            // zero slots after the logical row
+            const uint32_t tail = nch + wave * 64 + ln;
+            if (tail < nslots) ((uint4 *)row)[tail] = make_uint4(0u, 0u, 0u, 0u);
+        }
+    }
+    if (dig) {
+        dsum = wave_sum64(dsum);
+        if (ln == 0) atomicAdd((unsigned long long *)&T.digest[0], (unsigned long long)dsum);
+    }
+}
+
+// count0 digest (grid n, block 256)
+__global__ __launch_bounds__(256) void k_digest_count0(const TgtDev *__restrict__ tg) {
+    const TgtDev &T = tg[blockIdx.x];
+    if (!T.digest) return;
+    uint64_t s = 0;
+    for (uint32_t r = threadIdx.x; r < T.Mp; r += 256) s += dig_count0(r, T.count0[r]);
+    s = wave_sum64(s);
+    if (lane_id() == 0) atomicAdd((unsigned long long *)&T.digest[1], (unsigned long long)s);
+}
+
+// cref blocks -> condensed reference rows (for inspection).  grid (Mp, 1), block 256
+__global__ __launch_bounds__(256) void k_cref_rows(const TgtDev *__restrict__ tg, uint32_t t,
+                                                  uint64_t *__restrict__ rows, uint32_t rw) {
+    const TgtDev &T = tg[t];
+    const uint32_t r = blockIdx.x;
+    const uint16_t *blk = T.cref + (uint64_t)(r >> 4) * T.Kpad;
+    for (uint32_t w = threadIdx.x; w < rw; w += 256) {
+        uint64_t v = 0;
+        for (uint32_t q = 0; q < 64; q++) {
+            const uint32_t k = w * 64 + q;
+            if (k < T.K) v |= (uint64_t)((blk[k] >> (r & 15)) & 1u) << q;
+        }
+        rows[(uint64_t)r * rw + w] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_site_to_hap(const PanelDev &P, uint64_t *hap, hipStream_t s) {
+    dim3 grid((unsigned)((P.hap_stride + 3) / 4), (P.N_ad + 63) / 64);
+    hipLaunchKernelGGL(k_site_to_hap, grid, dim3(256), 0, s, P, hap);
+    return hipGetLastError();
+}
+hipError_t launch_panel_mono(const PanelDev &P, uint64_t *pand, uint64_t *por, hipStream_t s) {
+    hipLaunchKernelGGL(k_panel_mono, dim3((P.M + 3) / 4), dim3(256), 0, s, P, pand, por);
+    return hipGetLastError();
+}
+hipError_t launch_distance(const PanelDev &P, const TgtDev *tg, const uint32_t *tlist,
+                           uint32_t nsel, hipStream_t s) {
+    if (!nsel) return hipSuccess;
+    hipLaunchKernelGGL(k_distance, dim3((P.N_ad + 15) / 16, nsel), dim3(256), 0, s, P, tg, tlist);
+    return hipGetLastError();
+}
+hipError_t launch_select(const PanelDev &P, const TgtDev *tg, uint32_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_select, dim3(n), dim3(SEL_NT), 0, s, P, tg);
+    return hipGetLastError();
+}
+hipError_t launch_monos(const PanelDev &P, const TgtDev *tg, const uint32_t *tlist, uint32_t n,
+                        uint32_t max_ww, hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_monos, dim3((max_ww + 63) / 64, n), dim3(1024), 0, s, P, tg, tlist);
+    return hipGetLastError();
+}
+hipError_t launch_splits(const PanelDev &P, const TgtDev *tg, uint32_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_splits_clear, dim3(n), dim3(64), 0, s, tg);
+    hipLaunchKernelGGL(k_splits, dim3(n), dim3(64), 0, s, P, tg);
+    return hipGetLastError();
+}
+hipError_t launch_bits_to_list(const TgtDev *tg, uint32_t n, uint32_t nbits_bh, int which,
+                               hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_bits_to_list, dim3(n), dim3(SEL_NT), 0, s, tg, nbits_bh, which);
+    return hipGetLastError();
+}
+
+#define CREF_CASE(E)                                                                      \
+    if (ept <= E) {                                                                       \
+        hipLaunchKernelGGL(k_cref<E>, dim3(nitems), dim3(CREF_NT), lds, s, P, tg, items); \
+        return hipGetLastError();                                                         \
+    }
+hipError_t launch_cref(const PanelDev &P, const TgtDev *tg, const CrefItem *items, uint32_t nitems,
+                       uint32_t maxK, hipStream_t s) {
+    if (!nitems) return hipSuccess;
+    const uint32_t ept = (maxK + CREF_NT - 1) / CREF_NT;
+    const size_t lds = (size_t)((P.N_ad + 63) / 64) * sizeof(uint64_t);
+    CREF_CASE(4)
+    CREF_CASE(8)
+    CREF_CASE(16)
+    CREF_CASE(24)
+    CREF_CASE(32)
+    CREF_CASE(48)
+    return hipErrorInvalidValue;
+}
+#undef CREF_CASE
+
+uint32_t cref_max_k() { return 48 * CREF_NT; }
+
+hipError_t launch_digest_count0(const TgtDev *tg, uint32_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_digest_count0, dim3(n), dim3(256), 0, s, tg);
+    return hipGetLastError();
+}
+hipError_t launch_cref_rows(const TgtDev *tg, uint32_t t, uint32_t Mp, uint64_t *rows, uint32_t rw,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_cref_rows, dim3(Mp), dim3(256), 0, s, tg, t, rows, rw);
+    return hipGetLastError();
+}
+
+// PBWT variants: (threads, chunks per wave)
+struct PbwtVariant {
+    int nt, cpw;
+    void (*fn)(const TgtDev *, const PbwtItem *);
+};
+static const PbwtVariant kPbwtVariants[] = {
+    {256, 4, k_pbwt<256, 4>},    {256, 8, k_pbwt<256, 8>},    {256, 16, k_pbwt<256, 16>},
+    {256, 24, k_pbwt<256, 24>},  {256, 32, k_pbwt<256, 32>},  {256, 40, k_pbwt<256, 40>},
+    {512, 8, k_pbwt<512, 8>},    {512, 16, k_pbwt<512, 16>},  {512, 20, k_pbwt<512, 20>},
+    {512, 24, k_pbwt<512, 24>},  {512, 32, k_pbwt<512, 32>},  {1024, 8, k_pbwt<1024, 8>},
+    {1024, 12, k_pbwt<1024, 12>}, {1024, 16, k_pbwt<1024, 16>}, {1024, 24, k_pbwt<1024, 24>},
+};
+constexpr int kNumPbwtVariants = sizeof(kPbwtVariants) / sizeof(kPbwtVariants[0]);
+
+int pbwt_variant_count() { return kNumPbwtVariants; }
+int pbwt_variant_threads(int v) { return kPbwtVariants[v].nt; }
+int pbwt_variant_cpw(int v) { return kPbwtVariants[v].cpw; }
+uint32_t pbwt_variant_maxk(int v) {
+    return (uint32_t)(kPbwtVariants[v].nt / 64) * kPbwtVariants[v].cpw * 64;
+}
+size_t pbwt_lds_bytes(int v, uint32_t Kpad) {
+    return (size_t)Kpad * 4 + (size_t)Kpad * 2 + 2 * (kPbwtVariants[v].nt / 64) * 4;
+}
+hipError_t pbwt_prepare(int v, size_t lds) {
+    return hipFuncSetAttribute((const void *)kPbwtVariants[v].fn,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+hipError_t launch_pbwt(int v, const TgtDev *tg, const PbwtItem *items, uint32_t nitems, size_t lds,
+                       hipStream_t s) {
+    if (!nitems) return hipSuccess;
+    hipLaunchKernelGGL(kPbwtVariants[v].fn, dim3(nitems), dim3(kPbwtVariants[v].nt), lds, s, tg,
+                       items);
+    return hipGetLastError();
+}
+
+}  // namespace eimp

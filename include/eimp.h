@@ -1,0 +1,180 @@
+/*
+ * eimp.h -- C ABI of the MI355X (gfx950) engine for EagleImp's phasing
+ * pre-stage: top-K conditioning haplotypes, split/call sites, condensed
+ * reference and compressed forward/backward PBWT.
+ *
+ * Library: eagleimp_amd/libeimp.so.  Plain pointers and sizes only.  Every
+ * function returns EIMP_OK (0) or a negative status; eimp_last_error() gives
+ * the message of the last failure on the calling thread.
+ *
+ * Threading: an engine handle is used by one host thread at a time, exactly
+ * as the reference's GPUEngine is used by one producer thread
+ * (src/GPUHandler.cpp:148).  Calls are synchronous: when a call returns,
+ * its results are complete in the caller's buffers.
+ *
+ * Output layout (the reference's drop-in GPU layout, src/GPUHandler.cpp:259,
+ * src/GPUEngine.cu:160,182-211): per target 2*Mp rows of 4*Kwords uint32,
+ * Kwords = roundToMultiple(K,512)/64, forward rows r = 0..Mp-1 first, then the
+ * backward rows in processing order (row Mp + i holds site Mp-1-i).  Each row
+ * holds ceil(K/32) (data, cumulative zero count) pairs; the remaining pairs of
+ * a row are zero (uninitialised in the reference).
+ */
+#ifndef EIMP_H_
+#define EIMP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EIMP_OK 0
+#define EIMP_ERR_INVALID (-1)     /* bad argument or inconsistent sizes */
+#define EIMP_ERR_HIP (-2)         /* HIP runtime error */
+#define EIMP_ERR_NOMEM (-3)       /* device or host allocation failed */
+#define EIMP_ERR_STATE (-4)       /* call not valid in the engine's state */
+#define EIMP_ERR_UNSUPPORTED (-5) /* size beyond what the kernels support */
+
+typedef struct eimp_engine eimp_engine;
+
+/* Message of the last failure on this thread ("" if none). */
+const char *eimp_last_error(void);
+
+/* Number of visible HIP devices.
+ * Replaces hybridsys GPU discovery (src/hybridsys/Hybridsys.cpp:55-60,116-153). */
+int eimp_device_count(int *count);
+
+/* ---------------------------------------------------------------------- */
+/* Reference GPUEngine interface (src/GPUEngine.h:46-66)                    */
+/* ---------------------------------------------------------------------- */
+
+/* GPUEngine::GPUEngine(int index)  (src/GPUEngine.cu:13-42) */
+int eimp_engine_create(int device_index, eimp_engine **engine);
+
+/* ~GPUEngine; also releases what eimp_engine_free would. */
+int eimp_engine_destroy(eimp_engine *engine);
+
+/* GPUEngine::initialize(K, Kwords, M, Nrefhapscorr, NrefhapsAD, refTCapacity,
+ * refTData)  (src/GPUEngine.h:50, src/GPUEngine.cu:44-96).
+ * refTData: M site-major rows of refTCapacity bytes (refTCapacity a multiple of
+ * 64, bit h of row m = allele of haplotype h at site m, padding bits zero),
+ * uploaded once and kept resident.  Kwords must equal roundToMultiple(K,512)/64. */
+int eimp_engine_initialize(eimp_engine *engine, size_t K, size_t Kwords, size_t M,
+                           size_t Nrefhapscorr, size_t NrefhapsAD, size_t refTCapacity,
+                           const uint64_t *refTData);
+
+/* GPUEngine::getBlockSize()  (src/GPUEngine.h:53) -- targets per call. */
+int eimp_engine_block_size(eimp_engine *engine, size_t *block_size);
+
+/* GPUEngine::runCrefPBWTKernel(tgtData, splitdata, besthapdata, crefTOffsets,
+ * cPBWT, count0s)  (src/GPUEngine.h:55-61, src/GPUEngine.cu:98-217).
+ * For each of the n targets: tgtData[t] = GenotypeVector words over all M
+ * sites (roundToMultiple(2M,512)/8 bytes are read), splitdata[t] = split-site
+ * bitmap (roundToMultiple(M,512)/8 bytes), besthapdata[t] = best-haps bitmap
+ * (refTCapacity bytes, exactly K bits set).  crefTOffsets has n+1 entries, the
+ * prefix sums of Mp = 2*#splits+1.  cPBWT[t] receives 8*Mp*Kwords uint32
+ * (layout above) and count0s[t] receives Mp int32. */
+int eimp_engine_run_cref_pbwt(eimp_engine *engine, size_t n, const uint64_t *const *tgtData,
+                              const uint64_t *const *splitdata,
+                              const uint64_t *const *besthapdata, const uint32_t *crefTOffsets,
+                              uint32_t *const *cPBWT, int32_t *const *count0s);
+
+/* GPUEngine::free()  (src/GPUEngine.cu:219-225). */
+int eimp_engine_free(eimp_engine *engine);
+
+/* GPUEngine::getCUDAIndex()  (src/GPUEngine.h:66). */
+int eimp_engine_device_index(eimp_engine *engine, int *index);
+
+/* ---------------------------------------------------------------------- */
+/* Device-side Target::prepPhase (src/Target.cpp:95-163): new work -- the   */
+/* reference runs findBestHaps/findMonos/findCallLocs on the host           */
+/* (src/GPUHandler.cpp:119-136).                                            */
+/* ---------------------------------------------------------------------- */
+
+/* Genetic positions (float cM per site, src/VCFData.cpp:1189) and, optionally,
+ * the hap-major panel (src/VCFData.h:203; NrefhapsAD rows of hapStride uint64
+ * words, stride a multiple of 8, bit m = site m).  Without hapData the engine
+ * transposes the resident site-major panel on the device.  haploidRef: NULL
+ * or NrefhapsAD/2 bytes, nonzero = haploid reference sample whose second
+ * haplotype is skipped (src/Target.cpp:804-806,835). */
+int eimp_engine_set_panel_extras(eimp_engine *engine, const float *cM, const uint64_t *hapData,
+                                 size_t hapStride, const uint8_t *haploidRef);
+
+/* One target of a batch: window [s0, s0+W) of the panel's sites (the
+ * reference's chunk semantics), genotypes over the window as GenotypeVector
+ * words (2*roundToMultiple(W,512)/64 uint64, src/Datatypes.cpp:27-56), K
+ * (clamped to Ncorr as src/PBWTPhaser.cpp:683), ignoreSample (-1 none; the
+ * sample index skipped in top-K, src/Target.cpp:791-795). */
+typedef struct eimp_target {
+    const uint64_t *gt;
+    uint64_t s0;
+    uint64_t W;
+    uint64_t K;
+    int64_t ignoreSample;
+} eimp_target;
+
+/* Batch flags */
+#define EIMP_BATCH_DIGEST 1u      /* compute per-target output digests */
+#define EIMP_BATCH_COMPACT 4u     /* rows of 2*ceil(K/32) uint32 (CPU layout,
+                                     src/PBWT.cpp:36) instead of 4*Kwords */
+
+/* Per-target summary of a batch */
+typedef struct eimp_target_result {
+    uint64_t S;          /* split sites */
+    uint64_t Mp;         /* 2*S+1 PBWT rows per direction */
+    uint64_t ncalls;     /* call (het split) sites */
+    uint64_t K;          /* K after clamping */
+    uint64_t digest_pbwt;   /* sum of fmix32(v + g*0x9E3779B1) over logical words */
+    uint64_t digest_count0; /* sum of fmix32(c0 + r*0x7FEB352D + 0x68E31DA4) */
+} eimp_target_result;
+
+/* Whole pre-stage for n targets on the device: top-K by (nerr, index),
+ * monos, split/call sites, condensed reference, forward and backward PBWT.
+ * results[t] is filled for every target.  cMmaxSplit as src/Args.cpp:290.
+ * Outputs stay in the engine's device pool (the sink) until the next call. */
+int eimp_engine_run_batch(eimp_engine *engine, size_t n, const eimp_target *targets,
+                          float cMmaxSplit, uint32_t flags, eimp_target_result *results);
+
+/* Copies outputs of target t of the last batch (they stay on the device
+ * until the next batch or drop-in call).  Any pointer may be NULL.  Sizes: besthaps K uint32, bhflags
+ * roundToMultiple(NrefhapsAD,512)/64 uint64, splitsites S uint32, splits
+ * roundToMultiple(W,512)/64 uint64, cMdiffs S+1 float (0 if S==0), count0 Mp
+ * int32, pbwt 2*Mp*rowstride uint32, cref_rows Mp*roundToMultiple(K,512)/64
+ * uint64 (condensed reference rows, src/Target.cpp:253-262). */
+int eimp_engine_copy_target(eimp_engine *engine, size_t t, uint32_t *besthaps, uint64_t *bhflags,
+                            uint32_t *splitsites, uint64_t *splits, float *cMdiffs,
+                            int32_t *count0, uint32_t *pbwt, uint64_t *cref_rows);
+
+/* ---------------------------------------------------------------------- */
+/* Measurement                                                              */
+/* ---------------------------------------------------------------------- */
+
+/* Kernel classes timed with HIP events on the engine's stream. */
+#define EIMP_KCLASS_DISTANCE 0
+#define EIMP_KCLASS_SELECT 1
+#define EIMP_KCLASS_MONOS 2
+#define EIMP_KCLASS_SPLITS 3
+#define EIMP_KCLASS_CREF 4
+#define EIMP_KCLASS_PBWT 5
+#define EIMP_KCLASS_COPY 6
+#define EIMP_NKCLASS 7
+
+/* Enables (1) or disables (0) per-launch event timing. */
+int eimp_engine_set_timing(eimp_engine *engine, int enable);
+/* Accumulated milliseconds and launches per kernel class since the last reset. */
+int eimp_engine_get_timing(eimp_engine *engine, double *ms, uint64_t *launches);
+int eimp_engine_reset_timing(eimp_engine *engine);
+
+/* PBWT kernel variant: -1 automatic, otherwise an index into the engine's
+ * variant table (threads per workgroup x chunks per wave); the autotuner in
+ * the Python package sweeps these. */
+int eimp_engine_set_pbwt_variant(eimp_engine *engine, int variant);
+int eimp_pbwt_variant_count(void);
+int eimp_pbwt_variant_info(int variant, int *threads, int *chunks_per_wave, int *max_k);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EIMP_H_ */

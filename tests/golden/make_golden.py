@@ -1,0 +1,105 @@
+"""Writes the golden vectors in this folder from the reference's own code.
+
+Runs in the development container only (it needs /root/reference to build
+oracle/_ref/libeimp_ref.so, the reference's PBWT.cpp, Datatypes.cpp and
+RingBuffer.cpp compiled in place with oracle/ref_harness.cpp).  The fixtures
+it writes are plain data (inputs and the reference's outputs) in .npz files:
+
+  ref_pbwt.npz      condensed rows + count0 -> the reference's compressed
+                    forward/backward PBWT (src/PBWT.cpp:207-285) and
+                    mapInterval results (src/PBWT.cpp:288-382)
+  ref_genotype.npz  genotype codes -> the reference GenotypeVector words
+                    (src/Datatypes.cpp:27-56) and compare_withPreInit error
+                    counts (src/Datatypes.cpp:107-116)
+
+Usage:  python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+import oracle  # noqa: E402
+
+PBWT_CASES = [  # (K, Mp, density, seed)
+    (2, 1, 0.5, 1), (2, 9, 0.5, 2), (31, 7, 0.3, 3), (32, 33, 0.5, 4), (33, 33, 0.5, 5),
+    (63, 17, 0.2, 6), (64, 64, 0.5, 7), (65, 64, 0.7, 8), (100, 41, 0.5, 9),
+    (1000, 64, 0.5, 10), (513, 25, 0.05, 11), (777, 31, 0.95, 12), (96, 1, 0.5, 13),
+]
+
+
+def random_cref(K, Mp, dens, rng):
+    bits = (rng.random((Mp, K)) < dens).astype(np.uint8)
+    # a few structured rows: all zero, all one, alternating
+    if Mp > 3:
+        bits[1] = 0
+        bits[2] = 1
+        bits[3] = np.arange(K) % 2
+    kw = ((K + 511) // 512) * 8
+    padded = np.zeros((Mp, kw * 64), np.uint8)
+    padded[:, :K] = bits
+    cref = np.packbits(padded, axis=1, bitorder="little").view(np.uint64)
+    count0 = (K - bits.sum(axis=1)).astype(np.int32)
+    return cref, count0
+
+
+def pbwt_fixtures():
+    out = {}
+    for i, (K, Mp, dens, seed) in enumerate(PBWT_CASES):
+        rng = np.random.default_rng(seed)
+        cref, count0 = random_cref(K, Mp, dens, rng)
+        qs = []
+        for _ in range(24):
+            rev = int(rng.integers(0, 2))
+            m = int(rng.integers(0, Mp))
+            a, b = sorted(int(x) for x in rng.integers(0, K, size=2))
+            if rng.random() < 0.25:
+                b = a  # single element
+            if rng.random() < 0.1:
+                a, b = b + 1, b  # empty
+            qs.append((rev, m, a, b))
+        qs = np.asarray(qs, np.int32)
+        pbwt, qres = oracle.ref_build_pbwt(cref, K, count0, qs)
+        out[f"c{i}_K"] = np.int64(K)
+        out[f"c{i}_cref"] = cref
+        out[f"c{i}_count0"] = count0
+        out[f"c{i}_pbwt"] = pbwt
+        out[f"c{i}_queries"] = qs
+        out[f"c{i}_qres"] = qres
+    out["ncases"] = np.int64(len(PBWT_CASES))
+    np.savez_compressed(os.path.join(HERE, "ref_pbwt.npz"), **out)
+
+
+def genotype_fixtures():
+    out = {}
+    Ws = [1, 63, 64, 65, 511, 512, 513, 1000, 1537]
+    for i, W in enumerate(Ws):
+        rng = np.random.default_rng(100 + i)
+        codes = rng.integers(0, 4, size=W).astype(np.uint8)
+        if W > 10:
+            codes[:5] = [0, 1, 2, 3, 3]
+        hbits = (rng.random(W) < 0.5).astype(np.uint8)
+        hw = ((W + 511) // 512) * 8
+        padded = np.zeros(hw * 64, np.uint8)
+        padded[:W] = hbits
+        hap = np.packbits(padded, bitorder="little").view(np.uint64)
+        words, nerr = oracle.ref_genotype_compare(codes, hap, W)
+        out[f"g{i}_codes"] = codes
+        out[f"g{i}_hap"] = hap
+        out[f"g{i}_words"] = words
+        out[f"g{i}_nerr"] = np.int64(nerr)
+    out["ncases"] = np.int64(len(Ws))
+    np.savez_compressed(os.path.join(HERE, "ref_genotype.npz"), **out)
+
+
+if __name__ == "__main__":
+    oracle.build()
+    if not oracle.ref_available():
+        sys.exit("reference harness not built (is /root/reference present?)")
+    pbwt_fixtures()
+    genotype_fixtures()
+    for f in ("ref_pbwt.npz", "ref_genotype.npz"):
+        print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")
