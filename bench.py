@@ -1,0 +1,224 @@
+"""Benchmark of the phasing pre-stage (top-K select + condensed PBWT build) on one GPU.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 2]
+
+A step is one engine batch over `--batch` targets of the workload (synthetic
+panel and targets from eagleimp_amd.synth, inputs resident in HBM before the
+timed region).  Each batch writes every target's forward/backward compressed
+PBWT and count0 into the engine's HBM output pool (the sink) and a device
+digest of them; digests of a seeded sample of targets are checked against the
+CPU oracle.  One JSON line is printed.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "targets/sec (top-K select + cPBWT build) bit-exact; HBM GB/s % peak; p99 ms"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def pbwt_alg_bytes(Mp, K):
+    """Algorithmic HBM bytes of one PBWT launch item pair (fwd + bck) of a target:
+    the two compressed directions written (8 bytes per 32 haplotypes per row)
+    plus the condensed reference read by each direction (K bits per row)."""
+    Mp = np.asarray(Mp, np.float64)
+    K = np.asarray(K, np.float64)
+    return 2 * Mp * (8 * np.ceil(K / 32)) + 2 * Mp * np.ceil(K / 8)
+
+
+def b_alg(Mp, K, W, N, M, T):
+    """Whole pre-stage algorithmic bytes per target: cPBWT fwd+bck with offsets
+    and count0 out, genotypes in, two panel passes amortised over the run."""
+    Mp = np.asarray(Mp, np.float64)
+    K = np.asarray(K, np.float64)
+    W = np.asarray(W, np.float64)
+    return Mp * (16 * np.ceil(K / 32) + 4) + 2 * np.ceil(W / 512) * 64 + (2 * M * N / 8) / T
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", type=int, default=2, choices=[1, 2, 3, 4])
+    ap.add_argument("--targets", type=int, default=None, help="targets in the workload")
+    ap.add_argument("--batch", type=int, default=None, help="targets per step")
+    ap.add_argument("--variant", type=int, default=-1, help="PBWT kernel variant (-1 auto)")
+    ap.add_argument("--verify", type=int, default=2, help="targets checked against the oracle")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain", type=int, default=4, help="targets run one launch sequence each")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
+                          "(multi-GPU sharding is by target, see DESIGN.md)"}))
+        sys.exit(2)
+
+    from eagleimp_amd import synth
+    from eagleimp_amd.engine import Engine
+
+    cfg = dict(synth.WORKLOADS[args.workload])
+    T = args.targets or cfg["T"]
+    batch = args.batch or max(1, -(-T // args.steps))
+    need = (args.warmup + args.steps) * batch
+    log(f"workload {cfg['name']}: generating panel N={cfg['N']} M={cfg['M']} and {T} targets")
+    t0 = time.time()
+    panel = synth.make_panel(args.workload, cfg["N"], cfg["M"], cfg["F"], T, beta_a=cfg["beta_a"])
+    tg = synth.make_targets(panel, args.workload, T, K=cfg.get("K"), kset=cfg.get("kset"),
+                            wmin=cfg.get("wmin"), wmax=cfg.get("wmax"), miss=cfg["miss"],
+                            miss_hi=cfg.get("miss_hi"))
+    log(f"generated in {time.time() - t0:.1f}s")
+
+    eng = Engine(0)
+    eng.initialize(2, panel.M, panel.N, panel.N, panel.site)
+    eng.set_panel_extras(panel.cM, hap=panel.hap)
+    eng.set_pbwt_variant(args.variant)
+
+    def batch_idx(i):
+        return [(i * batch + j) % T for j in range(batch)]
+
+    def run(idx, digest=True):
+        return eng.run_batch([tg.gt[i] for i in idx], tg.s0[idx], tg.W[idx], tg.K[idx],
+                             digest=digest)
+
+    # warmup (also sizes the device pools)
+    for w in range(args.warmup):
+        log(f"warmup step {w}")
+        run(batch_idx(w))
+    eng.set_timing(True)
+    eng.reset_timing()
+    results = []
+    log(f"timed: {args.steps} steps x {batch} targets")
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        idx = batch_idx(args.warmup + s)
+        results.append((idx, run(idx)))
+    t_end = time.perf_counter()
+    wall = t_end - t_start
+    tim = eng.timing()
+    ntargets = args.steps * batch
+    tps = ntargets / wall
+    log(f"timed region {wall:.3f}s -> {tps:.1f} targets/s; kernels {tim}")
+
+    Mp_all = np.concatenate([r.Mp for _, r in results])
+    K_all = np.concatenate([r.K for _, r in results])
+    W_all = np.concatenate([tg.W[idx] for idx, _ in results])
+    pbwt_ms, pbwt_launches = tim["pbwt"]
+    pbwt_bytes = float(pbwt_alg_bytes(Mp_all, K_all).sum())
+    pbwt_avg_s = pbwt_ms / 1e3 / max(pbwt_launches, 1)
+    achieved = pbwt_bytes / max(pbwt_launches, 1) / pbwt_avg_s / 1e9 if pbwt_launches else 0.0
+    balg = float(b_alg(Mp_all, K_all, W_all, panel.N, panel.M, ntargets).sum())
+
+    # --- bit-exactness on a seeded sample against the oracle
+    bit_exact = None
+    if args.verify > 0:
+        import oracle
+        pv = oracle.PanelView(panel.hap, panel.site, panel.cM)
+        rng = np.random.default_rng(12345)
+        flat = [(idx[j], res, j) for idx, res in results for j in range(len(idx))]
+        pick = rng.choice(len(flat), size=min(args.verify, len(flat)), replace=False)
+        sample = [flat[p] for p in pick]
+        ref = oracle.process_batch(pv, [tg.gt[i] for i, _, _ in sample],
+                                   [tg.s0[i] for i, _, _ in sample], [tg.W[i] for i, _, _ in sample],
+                                   [tg.K[i] for i, _, _ in sample])
+        bit_exact = True
+        for n, (i, res, j) in enumerate(sample):
+            ok = (int(res.dig_pbwt[j]) == int(ref["dig_pbwt"][n])
+                  and int(res.dig_count0[j]) == int(ref["dig_count0"][n])
+                  and int(res.Mp[j]) == int(ref["Mp"][n]))
+            bit_exact &= ok
+        log(f"verified {len(sample)} targets against the oracle: bit_exact={bit_exact}")
+
+    # --- plain engine: one launch sequence per target
+    plain_tps = None
+    if args.plain > 0:
+        eng.set_timing(False)
+        pidx = batch_idx(0)[: args.plain]
+        run(pidx[:1])
+        t = time.perf_counter()
+        for i in pidx:
+            run([i])
+        plain_tps = len(pidx) / (time.perf_counter() - t)
+        log(f"plain one-target launches: {plain_tps:.2f} targets/s")
+
+    # --- CPU baseline (oracle, OpenMP over targets) on a bounded sample
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        pv = oracle.PanelView(panel.hap, panel.site, panel.cM)
+        cores = os.cpu_count() or 1
+        try:
+            cores = len(os.sched_getaffinity(0))
+        except Exception:
+            pass
+        cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+        i0 = [0]
+        t = time.perf_counter()
+        oracle.process_batch(pv, [tg.gt[0]], [tg.s0[0]], [tg.W[0]], [tg.K[0]], nthreads=1)
+        one = time.perf_counter() - t
+        n = int(max(1, min(T, cores * max(1, round(args.cpu_seconds / max(one, 1e-3))))))
+        n = max(min(n, T), 1)
+        ids = [i % T for i in range(1, n + 1)]
+        t = time.perf_counter()
+        oracle.process_batch(pv, [tg.gt[i] for i in ids], [tg.s0[i] for i in ids],
+                             [tg.W[i] for i in ids], [tg.K[i] for i in ids], nthreads=cores)
+        dt = time.perf_counter() - t
+        cpu = {"value": round(n / dt, 4), "unit": "targets/s", "cores": cores, "kind": "port",
+               "sample": f"{n} targets of the same workload (oracle C restatement, OpenMP "
+                         f"dynamic,1 over targets); 1 target on 1 core took {one:.2f}s"}
+        del i0
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(tps, 3),
+        "unit": "targets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": cfg["name"], "N": panel.N, "M": panel.M, "targets": T,
+            "targets_per_step": batch, "K": sorted(set(int(k) for k in K_all)),
+            "window": "full" if cfg.get("wmin") is None else "log-uniform",
+            "mean_Mp": round(float(Mp_all.mean()), 1),
+        },
+        "bit_exact": bit_exact,
+        "b_alg_GBps": round(balg / wall / 1e9, 2),
+        "b_alg_frac": round(balg / wall / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": {
+            "bound": "hbm", "kernel": "k_pbwt",
+            "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "avg_launch_ms": round(pbwt_avg_s * 1e3, 3),
+            "alg_bytes_per_launch": round(pbwt_bytes / max(pbwt_launches, 1)),
+        },
+        "kernel_ms": {k: round(v[0], 3) for k, v in tim.items()},
+        "plain_targets_per_s": None if plain_tps is None else round(plain_tps, 3),
+        "cpu_baseline": cpu,
+    }
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

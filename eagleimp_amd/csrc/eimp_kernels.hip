@@ -18,6 +18,8 @@
 //                    (Target::condensedReferenceT, src/Target.cpp:953-1059)
 //   k_pbwt           compressed forward / backward PBWT
 //                    (PBWT::buildComprPBWT, src/PBWT.cpp:207-285)
+#include <type_traits>
+
 #include "eimp_internal.h"
 
 namespace eimp {
@@ -518,11 +520,12 @@ __global__ __launch_bounds__(CREF_NT) void k_cref(PanelDev P, const TgtDev *__re
                 __syncthreads();
             }
         }
+        // haplotypes K..Kpad-1 get all-ones rows: the PBWT kernel's padding
         uint16_t *__restrict__ dst = T.cref + (uint64_t)b * Kpad;
 #pragma unroll
         for (int e = 0; e < EPT; e++) {
             const uint32_t k = tid + CREF_NT * e;
-            if (k < Kpad) dst[k] = (uint16_t)V[e];
+            if (k < Kpad) dst[k] = k < K ? (uint16_t)V[e] : (uint16_t)0xFFFFu;
         }
         __syncthreads();
         if (tid < 16) {
@@ -539,20 +542,47 @@ __global__ __launch_bounds__(CREF_NT) void k_cref(PanelDev P, const TgtDev *__re
 //
 // Position i of the current order is held by wave i/(64*CPW), chunk
 // (i/64)%CPW, lane i%64, as a 32-bit payload (hap index << 16 | the 16 bits
-// of the current cref block for that haplotype).  A step:
-//   pass 1  ballot of the row bit per chunk -> zero counts, wave totals to LDS
-//   pass 2  zeros-before / ones-before bases per chunk, destination of each
-//           payload under the stable partition (zeros first), payloads
-//           scattered into LDS, the ballot words + cumulative zero counts of
-//           the row written to HBM (one lane per chunk)
+// of the current cref block for that haplotype).  Positions K..NPOS-1 are
+// padding whose row bits are all ones: under the stable partition (zeros
+// first) they never move, and they give the 1-padding of the last word.
+// A step:
+//   pass 1  ballot of the row bit per chunk -> the chunk's two data words go
+//           to output lane c, zero counts summed per wave, wave totals to LDS
+//   pass 2  running zeros-before / ones-before bases per chunk and the
+//           destination of each payload under the partition; payloads are
+//           scattered into LDS; output lanes derive their cumulative zero
+//           counts with a wave scan and store the row
 //   pass 3  read back the payloads of my positions.
 // At each 16-row block boundary the next block's uint16 values are staged in
 // LDS and gathered by hap index.
 // ---------------------------------------------------------------------------
+// v_writelane_b32 through the LLVM intrinsic (no clang builtin in this
+// toolchain); the compiler sees it, so it inserts the wait states a VALU-written
+// SGPR operand needs.
+extern "C" __device__ uint32_t eimp_llvm_writelane(uint32_t val, uint32_t lane, uint32_t old)
+    __asm("llvm.amdgcn.writelane");
+
+// writes a wave-uniform value into lane L of a VGPR
+template <int L>
+__device__ __forceinline__ uint32_t writelane(uint32_t old, uint32_t val) {
+    return eimp_llvm_writelane(val, (uint32_t)L, old);
+}
+
+// compile-time loop: f(std::integral_constant<int, I>) for I in [B, E)
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
 template <int NT, int CPW>
 __global__ __launch_bounds__(NT) void k_pbwt(const TgtDev *__restrict__ tg,
                                             const PbwtItem *__restrict__ items) {
     constexpr int NW = NT / 64;
+    constexpr uint32_t NPOS = NW * CPW * 64;
+    static_assert(CPW <= 64, "one output lane per chunk");
     extern __shared__ uint32_t smem[];
     const PbwtItem it = items[blockIdx.x];
     const TgtDev &T = tg[it.t];
@@ -560,28 +590,34 @@ __global__ __launch_bounds__(NT) void k_pbwt(const TgtDev *__restrict__ tg,
     const uint32_t nch = Kpad / 64;
     const uint32_t npairs = (K + 31) / 32;
     const uint32_t KW = 2 * npairs;
-    uint32_t *pay = smem;                                    // Kpad
-    uint16_t *stage = (uint16_t *)(smem + Kpad);             // Kpad
-    uint32_t *tot = smem + Kpad + Kpad / 2;                  // 2*NW
-    const uint32_t wave = threadIdx.x >> 6, ln = lane_id();
+    uint32_t *pay = smem;                                    // NPOS
+    uint16_t *stage = (uint16_t *)(smem + NPOS);             // NPOS
+    uint32_t *tot = smem + NPOS + NPOS / 2;                  // 2*NW
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = lane_id();
     const bool rev = it.dir != 0;
     const uint16_t *__restrict__ cref = T.cref;
     const bool dig = T.digest != nullptr;
 
+    // haplotypes beyond Kpad read all-ones rows from the stage forever
+    for (uint32_t q = Kpad + threadIdx.x; q < NPOS; q += NT) stage[q] = 0xFFFFu;
     uint32_t p[CPW];
     {
-        const uint32_t r0 = rev ? Mp - 1 : 0;
-        const uint32_t b0 = r0 >> 4;
+        const uint32_t b0 = (rev ? Mp - 1 : 0) >> 4;
 #pragma unroll
         for (int c = 0; c < CPW; c++) {
             const uint32_t pos = (wave * CPW + c) * 64 + ln;
-            p[c] = pos < K ? (pos << 16) | (uint32_t)cref[(uint64_t)b0 * Kpad + pos] : 0u;
+            // the cref kernel writes all-ones for K <= k < Kpad
+            const uint32_t v = pos < Kpad ? (uint32_t)cref[(uint64_t)b0 * Kpad + pos] : 0xFFFFu;
+            p[c] = (pos << 16) | v;
         }
     }
     uint32_t *__restrict__ outbase = T.out + (rev ? (uint64_t)Mp * RS : 0ull);
     const uint64_t gbase = rev ? (uint64_t)Mp * KW : 0ull;
     uint64_t dsum = 0;
-    const uint32_t nslots = RS / 4;  // 4-word slots of a stored row (RS even)
+    const uint32_t nslots = RS / 4;  // 4-word slots of a stored row
+    const uint32_t och = wave * CPW + ln;  // chunk stored by this lane (ln < CPW)
+    const bool ostore = ln < (uint32_t)CPW && och < nch;
+    const uint32_t ln4 = 4 * ln;
 
     for (uint32_t step = 0; step < Mp; ++step) {
         const uint32_t r = rev ? Mp - 1 - step : step;
@@ -593,88 +629,76 @@ __global__ __launch_bounds__(NT) void k_pbwt(const TgtDev *__restrict__ tg,
             __syncthreads();
 #pragma unroll
             for (int c = 0; c < CPW; c++) {
-                const uint32_t pos = (wave * CPW + c) * 64 + ln;
-                if (pos < K) {
-                    const uint32_t id = p[c] >> 16;
-                    p[c] = (id << 16) | (uint32_t)stage[id];
-                }
+                const uint32_t id = p[c] >> 16;
+                p[c] = (id << 16) | (uint32_t)stage[id];
             }
         }
-        // pass 1
-        uint32_t zpre[CPW];
-        uint32_t zw = 0;
-#pragma unroll
-        for (int c = 0; c < CPW; c++) {
-            const uint32_t ch = wave * CPW + c;
-            zpre[c] = zw;
-            if (ch < nch) {
-                const uint32_t pos = ch * 64 + ln;
-                const bool bit = pos < K ? ((p[c] >> j) & 1u) : true;
-                const uint64_t m = __ballot(bit);
-                zw += 64u - (uint32_t)__popcll(m);
-            }
-        }
+        // pass 1: chunk c's ballot words go to lane c
+        uint32_t olo = 0, ohi = 0;
+        static_for<0, CPW>([&](auto ci) {
+            constexpr int c = decltype(ci)::value;
+            const uint64_t m = __ballot((p[c] >> j) & 1u);
+            olo = writelane<c>(olo, (uint32_t)m);
+            ohi = writelane<c>(ohi, (uint32_t)(m >> 32));
+        });
+        // zeros per chunk, and their prefix over this wave's chunks
+        const uint32_t zc = ln < (uint32_t)CPW ? 64u - (uint32_t)__popc(olo) - (uint32_t)__popc(ohi) : 0u;
+        const uint32_t zinc = wave_scan_incl(zc);
         const uint32_t par = step & 1u;
-        if (ln == 0) tot[par * NW + wave] = zw;
+        if (ln == 0) tot[par * NW + wave] = __builtin_amdgcn_readlane(zinc, 63);
         __syncthreads();
         uint32_t base = 0, cnt0 = 0;
 #pragma unroll
         for (int w2 = 0; w2 < NW; w2++) {
             const uint32_t v = tot[par * NW + w2];
             cnt0 += v;
-            if (w2 < (int)wave) base += v;
+            base += (w2 < (int)wave) ? v : 0u;
         }
-        // pass 2
-        uint32_t olo = 0, ohi = 0, ozb = 0;
+        base = __builtin_amdgcn_readfirstlane(base);
+        cnt0 = __builtin_amdgcn_readfirstlane(cnt0);
+        // pass 2: stable partition, zeros first; byte addresses into pay[]
+        uint32_t zrun4 = 4 * base;                          // zeros before the chunk, x4
+        uint32_t ob4 = 4 * (cnt0 + wave * CPW * 64 - base);  // ones before the chunk, x4
 #pragma unroll
         for (int c = 0; c < CPW; c++) {
-            const uint32_t ch = wave * CPW + c;
-            if (ch < nch) {
-                const uint32_t pos = ch * 64 + ln;
-                const bool valid = pos < K;
-                const bool bit = valid ? ((p[c] >> j) & 1u) : true;
-                const uint64_t m = __ballot(bit);
-                const uint32_t t1 = mbcnt(m);
-                const uint32_t zb = base + zpre[c];
-                const uint32_t ob = cnt0 + ch * 64 - zb;
-                const uint32_t dest = bit ? ob + t1 : zb + ln - t1;
-                if (valid) pay[dest] = p[c];
-                if (ln == (uint32_t)c) {
-                    olo = (uint32_t)m;
-                    ohi = (uint32_t)(m >> 32);
-                    ozb = zb;
-                }
-            }
+            uint32_t b = (p[c] >> j) & 1u;
+            // recompute the ballot here rather than keep CPW masks live across
+            // the barrier (they would spill)
+            asm volatile("" : "+v"(b));
+            const uint64_t m = __ballot(b != 0);
+            const uint32_t t1 = mbcnt(m);
+            const uint32_t zd4 = zrun4 + ln4 - 4 * t1;
+            const uint32_t od4 = ob4 + 4 * t1;
+            // unpredictable: keep a v_cndmask (a plain select may become a branch)
+            const uint32_t dest4 = __builtin_unpredictable(b != 0) ? od4 : zd4;
+            *(uint32_t *)((char *)pay + dest4) = p[c];
+            const uint32_t ones = (uint32_t)__popcll(m);
+            zrun4 += 4 * (64u - ones);
+            ob4 += 4 * ones;
         }
+        const uint32_t zb = base + zinc - zc;  // zeros before chunk (lanes < CPW)
         __syncthreads();
 #pragma unroll
-        for (int c = 0; c < CPW; c++) {
-            const uint32_t pos = (wave * CPW + c) * 64 + ln;
-            if (pos < K) p[c] = pay[pos];
-        }
+        for (int c = 0; c < CPW; c++) p[c] = pay[(wave * CPW + c) * 64 + ln];
         // the row: chunk ch holds pairs 2ch and 2ch+1
         uint32_t *__restrict__ row = outbase + (uint64_t)step * RS;
-        const uint64_t g0 = gbase + (uint64_t)step * KW;
-        {
-            const uint32_t ch = wave * CPW + ln;
-            if (ln < (uint32_t)CPW && ch < nch) {
-                const uint32_t o1 = ozb + 32u - (uint32_t)__popc(olo);
-                const uint32_t o3 = o1 + 32u - (uint32_t)__popc(ohi);
-                const bool second = 2 * ch + 1 < npairs;
-                uint2 *dst = (uint2 *)(row + 4 * ch);
-                dst[0] = make_uint2(olo, o1);
-                if (second) dst[1] = make_uint2(ohi, o3);
-                else if (4 * ch + 2 < RS) dst[1] = make_uint2(0u, 0u);
-                if (dig) {
-                    const uint64_t g = g0 + 4ull * ch;
-                    dsum += (uint64_t)dig_word(g, olo) + (uint64_t)dig_word(g + 1, o1);
-                    if (second) dsum += (uint64_t)dig_word(g + 2, ohi) + (uint64_t)dig_word(g + 3, o3);
-                }
+        if (ostore) {
+            const uint32_t o1 = zb + 32u - (uint32_t)__popc(olo);
+            const uint32_t o3 = o1 + 32u - (uint32_t)__popc(ohi);
+            const bool second = 2 * och + 1 < npairs;
+            uint2 *dst = (uint2 *)(row + 4 * och);
+            dst[0] = make_uint2(olo, o1);
+            if (second) dst[1] = make_uint2(ohi, o3);
+            else if (4 * och + 2 < RS) dst[1] = make_uint2(0u, 0u);
+            if (dig) {
+                const uint64_t g = gbase + (uint64_t)step * KW + 4ull * och;
+                dsum += (uint64_t)dig_word(g, olo) + (uint64_t)dig_word(g + 1, o1);
+                if (second) dsum += (uint64_t)dig_word(g + 2, ohi) + (uint64_t)dig_word(g + 3, o3);
             }
-            // zero slots after the logical row
-            const uint32_t tail = nch + wave * 64 + ln;
-            if (tail < nslots) ((uint4 *)row)[tail] = make_uint4(0u, 0u, 0u, 0u);
         }
+        // zero slots after the logical row
+        const uint32_t tail = nch + wave * 64 + ln;
+        if (tail < nslots) ((uint4 *)row)[tail] = make_uint4(0u, 0u, 0u, 0u);
     }
     if (dig) {
         dsum = wave_sum64(dsum);
@@ -804,7 +828,9 @@ uint32_t pbwt_variant_maxk(int v) {
     return (uint32_t)(kPbwtVariants[v].nt / 64) * kPbwtVariants[v].cpw * 64;
 }
 size_t pbwt_lds_bytes(int v, uint32_t Kpad) {
-    return (size_t)Kpad * 4 + (size_t)Kpad * 2 + 2 * (kPbwtVariants[v].nt / 64) * 4;
+    (void)Kpad;  // the kernel covers all NT/64*CPW*64 positions
+    const size_t npos = pbwt_variant_maxk(v);
+    return npos * 4 + npos * 2 + 2 * (kPbwtVariants[v].nt / 64) * 4;
 }
 hipError_t pbwt_prepare(int v, size_t lds) {
     return hipFuncSetAttribute((const void *)kPbwtVariants[v].fn,
