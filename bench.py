@@ -73,7 +73,7 @@ def main():
 
     cfg = dict(synth.WORKLOADS[args.workload])
     T = args.targets or cfg["T"]
-    batch = args.batch or max(1, -(-T // args.steps))
+    batch = args.batch or min(T, 256)
     need = (args.warmup + args.steps) * batch
     log(f"workload {cfg['name']}: generating panel N={cfg['N']} M={cfg['M']} and {T} targets")
     t0 = time.time()

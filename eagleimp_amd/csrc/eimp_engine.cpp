@@ -29,7 +29,7 @@ hipError_t launch_splits(const PanelDev &P, const TgtDev *tg, uint32_t n, hipStr
 hipError_t launch_bits_to_list(const TgtDev *tg, uint32_t n, uint32_t nbits_bh, int which,
                                hipStream_t s);
 hipError_t launch_cref(const PanelDev &P, const TgtDev *tg, const CrefItem *items, uint32_t nitems,
-                       uint32_t maxK, hipStream_t s);
+                       uint32_t maxK, bool ident, hipStream_t s);
 uint32_t cref_max_k();
 hipError_t launch_digest_count0(const TgtDev *tg, uint32_t n, hipStream_t s);
 hipError_t launch_cref_rows(const TgtDev *tg, uint32_t t, uint32_t Mp, uint64_t *rows, uint32_t rw,
@@ -219,15 +219,18 @@ struct eimp_engine {
                 return fail(EIMP_ERR_UNSUPPORTED, "PBWT variant too small for K");
             return pbwt_variant;
         }
-        // smallest chunk count per wave that covers K with 256 threads, else
-        // larger workgroups
+        // fewest chunks per wave among workgroups of up to 512 threads (measured
+        // best on MI355X at K = 10,000), 1024-thread workgroups only when needed
         int best = -1;
-        for (int v = 0; v < pbwt_variant_count(); v++) {
-            if (pbwt_variant_maxk(v) < Kmax_pad) continue;
-            if (best < 0) { best = v; continue; }
-            // prefer fewer threads, then fewer chunks per wave
-            int bt = pbwt_variant_threads(best), vt = pbwt_variant_threads(v);
-            if (vt < bt || (vt == bt && pbwt_variant_cpw(v) < pbwt_variant_cpw(best))) best = v;
+        for (int pass = 0; pass < 2 && best < 0; pass++) {
+            for (int v = 0; v < pbwt_variant_count(); v++) {
+                if (pbwt_variant_maxk(v) < Kmax_pad) continue;
+                if (pass == 0 && pbwt_variant_threads(v) > 512) continue;
+                if (best < 0 || pbwt_variant_cpw(v) < pbwt_variant_cpw(best) ||
+                    (pbwt_variant_cpw(v) == pbwt_variant_cpw(best) &&
+                     pbwt_variant_threads(v) < pbwt_variant_threads(best)))
+                    best = v;
+            }
         }
         if (best < 0) return fail(EIMP_ERR_UNSUPPORTED, "K beyond the PBWT kernels' LDS capacity");
         return best;
@@ -274,9 +277,8 @@ int run_build(eimp_engine *e, uint32_t n, bool digest, bool compact) {
     uint64_t cref_bytes = 0, c0_bytes = 0, out_bytes = 0;
     std::vector<uint64_t> cref_off(n), c0_off(n), out_off(n);
     uint32_t Kmax_pad = 0, Kmax = 0;
-    std::vector<CrefItem> citems;
+    std::vector<CrefItem> citems, citems_id;  // general / identity targets
     std::vector<PbwtItem> pitems;
-    const uint32_t bpi = 2;  // 16-row blocks per cref work item
     for (uint32_t t = 0; t < n; t++) {
         TgtDev &T = tg[t];
         T.Mp = 2 * T.S + 1;
@@ -290,9 +292,10 @@ int run_build(eimp_engine *e, uint32_t n, bool digest, bool compact) {
         out_off[t] = out_bytes;
         out_bytes += round_up(2ull * T.Mp * T.RS * 4, 256);
         Kmax_pad = std::max(Kmax_pad, T.Kpad);
-        Kmax = std::max(Kmax, T.K);
-        for (uint32_t b = 0; b < nblk; b += bpi)
-            citems.push_back(CrefItem{t, b, (uint32_t)std::min<uint64_t>(b + bpi, nblk), 0});
+        const bool ident = (T.flags & TF_IDENTITY) && e->Nad <= 2ull * 256 * 64;
+        if (!ident) Kmax = std::max(Kmax, T.K);
+        auto &ci = ident ? citems_id : citems;
+        for (uint32_t b = 0; b < nblk; b++) ci.push_back(CrefItem{t, b, b + 1, 0});
         pitems.push_back(PbwtItem{t, 0});
         pitems.push_back(PbwtItem{t, 1});
     }
@@ -314,7 +317,9 @@ int run_build(eimp_engine *e, uint32_t n, bool digest, bool compact) {
     CHK(e->d_count0.ensure(c0_bytes, "count0"));
     CHK(e->d_out.ensure(out_bytes, "cPBWT output"));
     CHK(e->d_digest.ensure((size_t)n * 16, "digest"));
-    CHK(e->d_citems.ensure(citems.size() * sizeof(CrefItem), "cref items"));
+    const size_t nci = citems.size();
+    citems.insert(citems.end(), citems_id.begin(), citems_id.end());
+    CHK(e->d_citems.ensure(std::max<size_t>(citems.size(), 1) * sizeof(CrefItem), "cref items"));
     CHK(e->d_pitems.ensure(pitems.size() * sizeof(PbwtItem), "pbwt items"));
     for (uint32_t t = 0; t < n; t++) {
         TgtDev &T = tg[t];
@@ -325,13 +330,16 @@ int run_build(eimp_engine *e, uint32_t n, bool digest, bool compact) {
     }
     if (digest) HIPCHK(hipMemsetAsync(e->d_digest.p, 0, (size_t)n * 16, e->stream));
     HIPCHK(hipMemcpyAsync(e->d_tg.p, tg.data(), n * sizeof(TgtDev), hipMemcpyHostToDevice, e->stream));
-    HIPCHK(hipMemcpyAsync(e->d_citems.p, citems.data(), citems.size() * sizeof(CrefItem),
-                          hipMemcpyHostToDevice, e->stream));
+    if (!citems.empty())
+        HIPCHK(hipMemcpyAsync(e->d_citems.p, citems.data(), citems.size() * sizeof(CrefItem),
+                              hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->d_pitems.p, pitems.data(), pitems.size() * sizeof(PbwtItem),
                           hipMemcpyHostToDevice, e->stream));
     int ti = e->tbegin(EIMP_KCLASS_CREF);
-    HIPCHK(launch_cref(e->pd, e->d_tg.as<TgtDev>(), e->d_citems.as<CrefItem>(),
-                       (uint32_t)citems.size(), Kmax, e->stream));
+    HIPCHK(launch_cref(e->pd, e->d_tg.as<TgtDev>(), e->d_citems.as<CrefItem>(), (uint32_t)nci,
+                       Kmax, false, e->stream));
+    HIPCHK(launch_cref(e->pd, e->d_tg.as<TgtDev>(), e->d_citems.as<CrefItem>() + nci,
+                       (uint32_t)(citems.size() - nci), 0, true, e->stream));
     e->tend(ti);
     ti = e->tbegin(EIMP_KCLASS_PBWT);
     HIPCHK(launch_pbwt(v, e->d_tg.as<TgtDev>(), e->d_pitems.as<PbwtItem>(), (uint32_t)pitems.size(),

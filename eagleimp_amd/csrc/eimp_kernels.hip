@@ -438,26 +438,82 @@ __global__ __launch_bounds__(SEL_NT) void k_bits_to_list(const TgtDev *__restric
 }
 
 // ---------------------------------------------------------------------------
-// a7: condensed reference.  One work item = a run of 16-row blocks of one
-// target.  Rows 2s (OR of the inconsistencies of segment s) and 2s+1 (the
-// haplotypes at split s) are gathered to the K chosen haplotypes; each thread
-// owns EPT haplotypes and accumulates their 16 bits of the block in a
-// register, written as uint16 per haplotype (transposed block).
-// grid nitems, block 512.
+// a7: condensed reference.  One work item = one 16-row block of a target
+// (segments 8b..8b+7).  Rows 2s (OR of the inconsistencies of segment s,
+// mergeInconsistencies, src/Target.cpp:1040-1059) and 2s+1 (the haplotypes at
+// split s, :986-1003) are first built over all N haplotypes in LDS, G
+// segments per fill, with every thread streaming its words of all the fill's
+// site rows (independent loads).  They are then reduced to the K chosen
+// haplotypes and transposed: one uint16 per haplotype holds the block's 16
+// rows.  Haplotypes K..Kpad-1 get all-ones values (the PBWT kernel's padding).
 // ---------------------------------------------------------------------------
-constexpr int CREF_NT = 512;
+constexpr int CREF_NT = 256;
+constexpr uint32_t CREF_LDS = 64 * 1024;  // bytes of staged rows per fill
 
+__host__ __device__ inline uint32_t cref_group(uint32_t nw) {
+    const uint32_t g = CREF_LDS / (16u * nw);  // 2 rows of nw words per segment
+    return g < 1 ? 1u : (g > 8 ? 8u : g);
+}
+
+// rows of segments [sa, sb) into srow[2*(s-sa)][.] (row 2(s-sa)+1 only if s < S)
+__device__ __forceinline__ void cref_fill(const PanelDev &P, const TgtDev &T, uint32_t sa,
+                                          uint32_t sb, uint32_t nw, uint64_t *srow) {
+    const uint64_t *__restrict__ site = P.site + T.s0 * P.site_stride;
+    const uint64_t ss = P.site_stride;
+    const uint32_t S = T.S, W = T.W;
+    const uint64_t lastmask = (P.N_ad & 63) ? ((1ull << (P.N_ad & 63)) - 1) : ~0ull;
+    // sites of the fill: after split sa-1 up to split sb-1 (or the window end)
+    const uint32_t m0 = sa == 0 ? 0u : T.splitsites[sa - 1] + 1;
+    const uint32_t m1 = sb <= S ? T.splitsites[sb - 1] + 1 : W;
+    for (uint32_t w = threadIdx.x; w < nw; w += CREF_NT) {
+        const uint64_t wmask = (w == nw - 1) ? lastmask : ~0ull;
+        uint32_t s = sa;
+        uint32_t next = s < S ? T.splitsites[s] : 0xFFFFFFFFu;
+        uint64_t a = 0;
+        // one pass over the sites, loads issued 4 at a time
+        for (uint32_t mb = m0; mb < m1; mb += 4) {
+            uint64_t v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t m = mb + u;
+                v[u] = m < m1 ? site[(uint64_t)m * ss + w] : 0ull;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t m = mb + u;
+                if (m >= m1) break;
+                if (m == next) {  // split: close segment s, its haplotypes are row 2s+1
+                    srow[(uint64_t)(2 * (s - sa)) * nw + w] = a & wmask;
+                    srow[(uint64_t)(2 * (s - sa) + 1) * nw + w] = v[u];
+                    a = 0;
+                    s++;
+                    next = s < S ? T.splitsites[s] : 0xFFFFFFFFu;
+                } else {
+                    const uint64_t g0 = T.gt[2 * (m >> 6)], g2 = T.gt[2 * (m >> 6) + 1];
+                    const uint64_t is0 = (g0 >> (m & 63)) & 1ull, is2 = (g2 >> (m & 63)) & 1ull;
+                    // HomRef: the ones are inconsistent; HomAlt: the zeros; het/missing: none
+                    const uint64_t flip = is2 ? ~0ull : 0ull;
+                    const uint64_t keep = (is0 ^ is2) ? ~0ull : 0ull;
+                    a |= (v[u] ^ flip) & keep;
+                }
+            }
+        }
+        if (sb == S + 1) srow[(uint64_t)(2 * (S - sa)) * nw + w] = a & wmask;  // last segment
+    }
+}
+
+// K chosen haplotypes by index (any target).  EPT haplotypes per thread.
 template <int EPT>
 __global__ __launch_bounds__(CREF_NT) void k_cref(PanelDev P, const TgtDev *__restrict__ tg,
                                                  const CrefItem *__restrict__ items) {
-    extern __shared__ uint64_t sm64[];
+    extern __shared__ uint64_t srow[];
+    __shared__ uint32_t cnt[16];
     const CrefItem it = items[blockIdx.x];
     const TgtDev &T = tg[it.t];
-    const uint32_t tid = threadIdx.x, wave = tid >> 6, l = lane_id();
-    const uint32_t K = T.K, S = T.S, W = T.W, Kpad = T.Kpad, Mp = T.Mp;
+    const uint32_t tid = threadIdx.x, l = lane_id();
+    const uint32_t K = T.K, S = T.S, Kpad = T.Kpad, Mp = T.Mp;
     const uint32_t nw = (P.N_ad + 63) / 64;
-    uint64_t *acc = sm64;            // nw words
-    __shared__ uint32_t cnt[16];
+    const uint32_t G = cref_group(nw);
     const bool ident = (T.flags & TF_IDENTITY) != 0;
     uint32_t idx[EPT];
 #pragma unroll
@@ -465,75 +521,119 @@ __global__ __launch_bounds__(CREF_NT) void k_cref(PanelDev P, const TgtDev *__re
         const uint32_t k = tid + CREF_NT * e;
         idx[e] = k < K ? (ident ? k : T.bh[k]) : 0u;
     }
-    const uint64_t *__restrict__ site = P.site + T.s0 * P.site_stride;
     for (uint32_t b = it.b0; b < it.b1; b++) {
         uint32_t V[EPT];
 #pragma unroll
         for (int e = 0; e < EPT; e++) V[e] = 0;
         if (tid < 16) cnt[tid] = 0;
-        for (uint32_t jj = 0; jj < 8; jj++) {
-            const uint32_t s = 8 * b + jj;
-            if (s > S) break;
-            // --- row 2s: inconsistencies of segment s (mergeInconsistencies, :1040-1059)
-            const uint32_t lo = s == 0 ? 0u : T.splitsites[s - 1] + 1;
-            const uint32_t hi = s < S ? T.splitsites[s] : W;
-            for (uint32_t w = tid; w < nw; w += CREF_NT) {
-                uint64_t a = 0;
-                for (uint32_t m = lo; m < hi; m++) {
-                    const uint64_t g0 = T.gt[2 * (m >> 6)], g2 = T.gt[2 * (m >> 6) + 1];
-                    const bool is0 = (g0 >> (m & 63)) & 1ull, is2 = (g2 >> (m & 63)) & 1ull;
-                    if (is0 == is2) continue;  // het or missing: consistent with all
-                    const uint64_t v = site[(uint64_t)m * P.site_stride + w];
-                    a |= is0 ? v : ~v;
-                }
-                acc[w] = a;
-            }
+        const uint32_t send = min(8 * b + 8, S + 1);
+        for (uint32_t sa = 8 * b; sa < send; sa += G) {
+            const uint32_t sb = min(sa + G, send);
+            cref_fill(P, T, sa, sb, nw, srow);
             __syncthreads();
-            {
-                const uint32_t rb = 2 * jj;
+            const uint32_t nrows = 2 * (sb - sa) - (sb == S + 1 ? 1u : 0u);
+            for (uint32_t rr = 0; rr < nrows; rr++) {
+                const uint32_t rb = 2 * (sa - 8 * b) + rr;
+                const uint64_t *row = srow + (uint64_t)rr * nw;
                 uint32_t ones = 0;
 #pragma unroll
                 for (int e = 0; e < EPT; e++) {
                     const uint32_t k = tid + CREF_NT * e;
-                    const uint32_t bit = k < K ? (uint32_t)((acc[idx[e] >> 6] >> (idx[e] & 63)) & 1ull) : 0u;
+                    const uint32_t bit = k < K ? (uint32_t)((row[idx[e] >> 6] >> (idx[e] & 63)) & 1ull) : 0u;
                     V[e] |= bit << rb;
                     ones += (uint32_t)__popcll(__ballot(bit));
                 }
                 if (l == 0) atomicAdd(&cnt[rb], ones);
             }
             __syncthreads();
-            if (s < S) {
-                // --- row 2s+1: haplotypes at split s (:986-1003)
-                const uint64_t *__restrict__ row = site + (uint64_t)T.splitsites[s] * P.site_stride;
-                for (uint32_t w = tid; w < nw; w += CREF_NT) acc[w] = row[w];
-                __syncthreads();
-                const uint32_t rb = 2 * jj + 1;
-                uint32_t ones = 0;
-#pragma unroll
-                for (int e = 0; e < EPT; e++) {
-                    const uint32_t k = tid + CREF_NT * e;
-                    const uint32_t bit = k < K ? (uint32_t)((acc[idx[e] >> 6] >> (idx[e] & 63)) & 1ull) : 0u;
-                    V[e] |= bit << rb;
-                    ones += (uint32_t)__popcll(__ballot(bit));
-                }
-                if (l == 0) atomicAdd(&cnt[rb], ones);
-                __syncthreads();
-            }
         }
-        // haplotypes K..Kpad-1 get all-ones rows: the PBWT kernel's padding
         uint16_t *__restrict__ dst = T.cref + (uint64_t)b * Kpad;
 #pragma unroll
         for (int e = 0; e < EPT; e++) {
             const uint32_t k = tid + CREF_NT * e;
             if (k < Kpad) dst[k] = k < K ? (uint16_t)V[e] : (uint16_t)0xFFFFu;
         }
-        __syncthreads();
         if (tid < 16) {
             const uint32_t r = 16 * b + tid;
             if (r < Mp) T.count0[r] = (int32_t)K - (int32_t)cnt[tid];
         }
         __syncthreads();
-        (void)wave;
+    }
+}
+
+// All N haplotypes in panel order (TF_IDENTITY): no gather.  Each thread
+// owns GPT groups of 64 haplotypes, reads their word of every staged row and
+// transposes into 64 uint16 values, written as 128 contiguous bytes.
+template <int GPT>
+__global__ __launch_bounds__(CREF_NT) void k_cref_ident(PanelDev P, const TgtDev *__restrict__ tg,
+                                                       const CrefItem *__restrict__ items) {
+    extern __shared__ uint64_t srow[];
+    __shared__ uint32_t cnt[16];
+    const CrefItem it = items[blockIdx.x];
+    const TgtDev &T = tg[it.t];
+    const uint32_t tid = threadIdx.x, l = lane_id();
+    const uint32_t K = T.K, S = T.S, Kpad = T.Kpad, Mp = T.Mp;
+    const uint32_t nw = (P.N_ad + 63) / 64;  // == Kpad / 64 (K == N_ad)
+    const uint32_t G = cref_group(nw);
+    for (uint32_t b = it.b0; b < it.b1; b++) {
+        uint32_t V[GPT][32];  // two uint16 values per register
+#pragma unroll
+        for (int e = 0; e < GPT; e++)
+#pragma unroll
+            for (int i = 0; i < 32; i++) V[e][i] = 0;
+        if (tid < 16) cnt[tid] = 0;
+        const uint32_t send = min(8 * b + 8, S + 1);
+        for (uint32_t sa = 8 * b; sa < send; sa += G) {
+            const uint32_t sb = min(sa + G, send);
+            cref_fill(P, T, sa, sb, nw, srow);
+            __syncthreads();
+            const uint32_t nrows = 2 * (sb - sa) - (sb == S + 1 ? 1u : 0u);
+            for (uint32_t rr = 0; rr < nrows; rr++) {
+                const uint32_t rb = 2 * (sa - 8 * b) + rr;
+                const uint64_t *row = srow + (uint64_t)rr * nw;
+                uint32_t ones = 0;
+#pragma unroll
+                for (int e = 0; e < GPT; e++) {
+                    const uint32_t q = tid + CREF_NT * e;
+                    const uint64_t wv = q < nw ? row[q] : 0ull;
+                    ones += (uint32_t)__popcll(wv);
+                    const uint32_t lo = (uint32_t)wv, hi = (uint32_t)(wv >> 32);
+#pragma unroll
+                    for (int i = 0; i < 16; i++) {
+                        // haplotypes 2i, 2i+1 (low word) and 32+2i, 33+2i (high word)
+                        V[e][i] |= (((lo >> (2 * i)) & 1u) << rb) | (((lo >> (2 * i + 1)) & 1u) << (16 + rb));
+                        V[e][16 + i] |= (((hi >> (2 * i)) & 1u) << rb) | (((hi >> (2 * i + 1)) & 1u) << (16 + rb));
+                    }
+                }
+                ones = wave_sum(ones);
+                if (l == 0) atomicAdd(&cnt[rb], ones);
+            }
+            __syncthreads();
+        }
+        uint16_t *__restrict__ dst = T.cref + (uint64_t)b * Kpad;
+#pragma unroll
+        for (int e = 0; e < GPT; e++) {
+            const uint32_t q = tid + CREF_NT * e;
+            if (q < nw) {
+                if (64 * q + 64 > K) {  // last group: haplotypes >= K are padding
+#pragma unroll
+                    for (int i = 0; i < 32; i++) {
+                        const uint32_t k0 = 64 * q + 2 * i;
+                        if (k0 >= K) V[e][i] |= 0xFFFFu;
+                        if (k0 + 1 >= K) V[e][i] |= 0xFFFF0000u;
+                    }
+                }
+                uint4 *d4 = (uint4 *)(dst + 64 * q);
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+                    d4[i] = make_uint4(V[e][4 * i], V[e][4 * i + 1], V[e][4 * i + 2], V[e][4 * i + 3]);
+            }
+        }
+        if (tid < 16) {
+            const uint32_t r = 16 * b + tid;
+            if (r < Mp) T.count0[r] = (int32_t)K - (int32_t)cnt[tid];
+        }
+        __syncthreads();
     }
 }
 
@@ -780,21 +880,34 @@ hipError_t launch_bits_to_list(const TgtDev *tg, uint32_t n, uint32_t nbits_bh, 
         return hipGetLastError();                                                         \
     }
 hipError_t launch_cref(const PanelDev &P, const TgtDev *tg, const CrefItem *items, uint32_t nitems,
-                       uint32_t maxK, hipStream_t s) {
+                       uint32_t maxK, bool ident, hipStream_t s) {
     if (!nitems) return hipSuccess;
+    const uint32_t nw = (P.N_ad + 63) / 64;
+    const size_t lds = (size_t)2 * cref_group(nw) * nw * sizeof(uint64_t);
+    if (ident) {
+        const uint32_t gpt = (nw + CREF_NT - 1) / CREF_NT;
+        if (gpt <= 1) {
+            hipLaunchKernelGGL(k_cref_ident<1>, dim3(nitems), dim3(CREF_NT), lds, s, P, tg, items);
+            return hipGetLastError();
+        }
+        if (gpt <= 2) {
+            hipLaunchKernelGGL(k_cref_ident<2>, dim3(nitems), dim3(CREF_NT), lds, s, P, tg, items);
+            return hipGetLastError();
+        }
+    }
     const uint32_t ept = (maxK + CREF_NT - 1) / CREF_NT;
-    const size_t lds = (size_t)((P.N_ad + 63) / 64) * sizeof(uint64_t);
     CREF_CASE(4)
     CREF_CASE(8)
     CREF_CASE(16)
-    CREF_CASE(24)
     CREF_CASE(32)
     CREF_CASE(48)
+    CREF_CASE(64)
+    CREF_CASE(96)
     return hipErrorInvalidValue;
 }
 #undef CREF_CASE
 
-uint32_t cref_max_k() { return 48 * CREF_NT; }
+uint32_t cref_max_k() { return 96 * CREF_NT; }
 
 hipError_t launch_digest_count0(const TgtDev *tg, uint32_t n, hipStream_t s) {
     if (!n) return hipSuccess;
