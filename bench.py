@@ -48,6 +48,32 @@ def b_alg(Mp, K, W, N, M, T):
     return Mp * (16 * np.ceil(K / 32) + 4) + 2 * np.ceil(W / 512) * 64 + (2 * M * N / 8) / T
 
 
+def rocpd_summary(dbs):
+    """Markdown tables from rocprofv3's SQLite output: per-kernel durations from
+    a kernel trace, per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) from counter passes."""
+    import sqlite3
+    for db in dbs:
+        con = sqlite3.connect(db)
+        tables = {r[0] for r in con.execute("select name from sqlite_master")}
+        print(f"### {os.path.basename(db)}\n")
+        rows = con.execute("select kernel_name, counter_name, count(*), avg(value), min(value), "
+                           "max(value) from counters_collection group by kernel_name, counter_name "
+                           "order by avg(value) desc").fetchall() if "counters_collection" in tables else []
+        if rows:
+            print("| kernel | counter | dispatches | mean KiB/dispatch | min | max |")
+            print("|---|---|---|---|---|---|")
+            for k, c, n, a, lo, hi in rows:
+                print(f"| `{k[:90]}` | {c} | {n} | {a:.1f} | {lo:.1f} | {hi:.1f} |")
+        else:
+            rows = con.execute("select name, total_calls, total_duration, average, percentage "
+                               "from top_kernels").fetchall()
+            print("| kernel | calls | total us | mean us | % |")
+            print("|---|---|---|---|---|")
+            for k, n, tot, avg, pct in rows:
+                print(f"| `{k[:90]}` | {n} | {tot:.1f} | {avg:.1f} | {pct:.2f} |")
+        print()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,7 +87,27 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain", type=int, default=4, help="targets run one launch sequence each")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES (scales HBM counters)")
+    ap.add_argument("--rocpd-summary", nargs="+", metavar="DB",
+                    help="summarise rocprofv3 result databases (kernel trace and/or "
+                         "FETCH_SIZE / WRITE_SIZE passes) and exit")
     args = ap.parse_args()
+
+    if args.rocpd_summary:
+        rocpd_summary(args.rocpd_summary)
+        return
+
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration
+        x = torch.ones(n // 4, dtype=torch.int32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n}), flush=True)
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
@@ -122,6 +168,13 @@ def main():
     pbwt_avg_s = pbwt_ms / 1e3 / max(pbwt_launches, 1)
     achieved = pbwt_bytes / max(pbwt_launches, 1) / pbwt_avg_s / 1e9 if pbwt_launches else 0.0
     balg = float(b_alg(Mp_all, K_all, W_all, panel.N, panel.M, ntargets).sum())
+    # HBM bytes per launch measured with rocprofv3 counter passes on this workload
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_pbwt_traffic.json")
+    if os.path.exists(tpath):
+        tr = json.load(open(tpath))
+        if tr.get("workload") == cfg["name"] and tr.get("targets_per_step") == batch:
+            traffic = round(float(tr["traffic_bytes_per_launch"]))
 
     # --- bit-exactness on a seeded sample against the oracle
     bit_exact = None
@@ -208,7 +261,7 @@ def main():
         "roofline": {
             "bound": "hbm", "kernel": "k_pbwt",
             "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "avg_launch_ms": round(pbwt_avg_s * 1e3, 3),
             "alg_bytes_per_launch": round(pbwt_bytes / max(pbwt_launches, 1)),
         },
