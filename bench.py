@@ -48,6 +48,13 @@ def b_alg(Mp, K, W, N, M, T):
     return Mp * (16 * np.ceil(K / 32) + 4) + 2 * np.ceil(W / 512) * 64 + (2 * M * N / 8) / T
 
 
+def _take(res, keep):
+    from eagleimp_amd.engine import BatchResult
+    k = np.asarray(keep, dtype=np.int64)
+    return BatchResult(S=res.S[k], Mp=res.Mp[k], ncalls=res.ncalls[k], K=res.K[k],
+                       dig_pbwt=res.dig_pbwt[k], dig_count0=res.dig_count0[k])
+
+
 def rocpd_summary(dbs):
     """Markdown tables from rocprofv3's SQLite output: per-kernel durations from
     a kernel trace, per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) from counter passes."""
@@ -87,6 +94,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain", type=int, default=4, help="targets run one launch sequence each")
+    ap.add_argument("--engine", choices=["stream", "batch"], default="stream",
+                    help="stream contexts (device-driven, queue kernels) or the synchronous "
+                         "batch engine")
+    ap.add_argument("--contexts", type=int, default=2, help="stream contexts in flight")
+    ap.add_argument("--graph-steps", type=int, default=2,
+                    help="steps replayed through captured hipGraphs after the timed region")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (scales HBM counters)")
     ap.add_argument("--rocpd-summary", nargs="+", metavar="DB",
@@ -141,36 +154,115 @@ def main():
         return eng.run_batch([tg.gt[i] for i in idx], tg.s0[idx], tg.W[idx], tg.K[idx],
                              digest=digest)
 
-    # warmup (also sizes the device pools)
-    for w in range(args.warmup):
-        log(f"warmup step {w}")
-        run(batch_idx(w))
-    eng.set_timing(True)
-    eng.reset_timing()
-    results = []
-    log(f"timed: {args.steps} steps x {batch} targets")
-    t_start = time.perf_counter()
-    for s in range(args.steps):
-        idx = batch_idx(args.warmup + s)
-        results.append((idx, run(idx)))
-    t_end = time.perf_counter()
-    wall = t_end - t_start
-    tim = eng.timing()
-    ntargets = args.steps * batch
+    graph_tps = None
+    if args.engine == "batch":
+        # synchronous engine: one run_batch call per step
+        for w in range(args.warmup):
+            log(f"warmup step {w}")
+            run(batch_idx(w))
+        eng.set_timing(True)
+        eng.reset_timing()
+        results = []
+        log(f"timed: {args.steps} steps x {batch} targets (batch engine)")
+        t_start = time.perf_counter()
+        for s in range(args.steps):
+            idx = batch_idx(args.warmup + s)
+            results.append((idx, run(idx)))
+        wall = time.perf_counter() - t_start
+        tim = eng.timing()
+        pbwt_ms, pbwt_launches = tim["pbwt"]
+    else:
+        free, total = eng.mem_info()
+        pool = int(max(free - (24 << 30), 1 << 30) // args.contexts)
+        maxW = int(tg.W.max())
+        maxK = int(min(int(tg.K.max()), panel.N))
+        log(f"stream contexts: {args.contexts} x pool {pool / 2**30:.1f} GiB "
+            f"(device free {free / 2**30:.1f} of {total / 2**30:.1f} GiB)")
+
+        def pipeline(sts, steps, first):
+            """steps batches over the contexts, round robin; deferred targets are
+            submitted again.  Returns [(idx, result)], per-class ms summed."""
+            todo = [batch_idx(first + s) for s in range(steps)]
+            out, kms = [], {}
+            inflight = {}
+            carry = []
+
+            def next_batch():
+                nonlocal carry
+                if carry:
+                    b, carry = carry[:batch], carry[batch:]
+                    return b
+                return todo.pop(0) if todo else None
+
+            def submit(ci):
+                b = next_batch()
+                if b is None:
+                    return
+                sts[ci].submit([tg.gt[i] for i in b], tg.s0[b], tg.W[b], tg.K[b])
+                inflight[ci] = b
+
+            for ci in range(len(sts)):
+                submit(ci)
+            ci = 0
+            while inflight:
+                if ci in inflight:
+                    b = inflight.pop(ci)
+                    res, nd, _ = sts[ci].wait()
+                    if sts[ci].timed:
+                        for k, v in sts[ci].kernel_ms().items():
+                            kms[k] = kms.get(k, 0.0) + v
+                    keep = [j for j in range(len(b)) if int(res.Mp[j]) > 0]
+                    if nd:
+                        carry += [b[j] for j in range(len(b)) if int(res.Mp[j]) == 0]
+                    out.append(([b[j] for j in keep], _take(res, keep)))
+                    submit(ci)
+                ci = (ci + 1) % len(sts)
+            return out, kms
+
+        def contexts(timed):
+            return [eng.stream(batch, maxW, maxK, pool, graph=not timed, timed=timed)
+                    for _ in range(args.contexts)]
+
+        sts = contexts(timed=True)
+        log(f"warmup: {args.warmup} steps")
+        pipeline(sts, args.warmup, 0)
+        log(f"timed: {args.steps} steps x {batch} targets ({args.contexts} stream contexts, "
+            f"events between kernel classes)")
+        t_start = time.perf_counter()
+        results, kms = pipeline(sts, args.steps, args.warmup)
+        wall = time.perf_counter() - t_start
+        for st in sts:
+            st.close()
+        tim = {k: (v, 0) for k, v in kms.items()}
+        # one PBWT launch per submission
+        pbwt_launches = sum(1 for _ in results)
+        pbwt_ms = kms.get("pbwt", 0.0)
+        tim["pbwt"] = (pbwt_ms, pbwt_launches)
+        if args.graph_steps > 0:
+            gsts = contexts(timed=False)
+            pipeline(gsts, 1, 0)
+            t = time.perf_counter()
+            gres, _ = pipeline(gsts, args.graph_steps, args.warmup)
+            gwall = time.perf_counter() - t
+            graph_tps = sum(len(i) for i, _ in gres) / gwall
+            for st in gsts:
+                st.close()
+            log(f"hipGraph replay: {graph_tps:.1f} targets/s")
+    ntargets = sum(len(idx) for idx, _ in results)
     tps = ntargets / wall
     log(f"timed region {wall:.3f}s -> {tps:.1f} targets/s; kernels {tim}")
 
     Mp_all = np.concatenate([r.Mp for _, r in results])
     K_all = np.concatenate([r.K for _, r in results])
     W_all = np.concatenate([tg.W[idx] for idx, _ in results])
-    pbwt_ms, pbwt_launches = tim["pbwt"]
     pbwt_bytes = float(pbwt_alg_bytes(Mp_all, K_all).sum())
     pbwt_avg_s = pbwt_ms / 1e3 / max(pbwt_launches, 1)
     achieved = pbwt_bytes / max(pbwt_launches, 1) / pbwt_avg_s / 1e9 if pbwt_launches else 0.0
     balg = float(b_alg(Mp_all, K_all, W_all, panel.N, panel.M, ntargets).sum())
+    kernel_name = "k_pbwt" if args.engine == "batch" else "k_pbwt_persist"
     # HBM bytes per launch measured with rocprofv3 counter passes on this workload
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "k_pbwt_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", f"{kernel_name}_traffic.json")
     if os.path.exists(tpath):
         tr = json.load(open(tpath))
         if tr.get("workload") == cfg["name"] and tr.get("targets_per_step") == batch:
@@ -259,7 +351,7 @@ def main():
         "b_alg_GBps": round(balg / wall / 1e9, 2),
         "b_alg_frac": round(balg / wall / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {
-            "bound": "hbm", "kernel": "k_pbwt",
+            "bound": "hbm", "kernel": kernel_name,
             "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "avg_launch_ms": round(pbwt_avg_s * 1e3, 3),
@@ -267,6 +359,8 @@ def main():
         },
         "kernel_ms": {k: round(v[0], 3) for k, v in tim.items()},
         "plain_targets_per_s": None if plain_tps is None else round(plain_tps, 3),
+        "graph_targets_per_s": None if graph_tps is None else round(graph_tps, 3),
+        "engine": args.engine,
         "cpu_baseline": cpu,
     }
     eng.close()

@@ -42,6 +42,19 @@ size_t pbwt_lds_bytes(int v, uint32_t Kpad);
 hipError_t pbwt_prepare(int v, size_t lds);
 hipError_t launch_pbwt(int v, const TgtDev *tg, const PbwtItem *items, uint32_t nitems, size_t lds,
                        hipStream_t s);
+hipError_t launch_prep_dev(int cls, const PanelDev &P, const TgtDev *tg, const uint32_t *d_n,
+                           uint32_t nmax, uint32_t max_ww, hipStream_t s);
+hipError_t launch_plan(TgtDev *tg, BatchDev *B, const PlanArgs &A, hipStream_t s);
+hipError_t launch_digest_count0_dev(const TgtDev *tg, const uint32_t *d_n, uint32_t nmax,
+                                    hipStream_t s);
+size_t cref_lds_bytes(const PanelDev &P);
+hipError_t cref_persist_prepare(const PanelDev &P, uint32_t maxK, int *occ);
+hipError_t launch_cref_persist(const PanelDev &P, const TgtDev *tg, BatchDev *B,
+                               const uint32_t *blockpref, uint32_t maxK, uint32_t grid,
+                               hipStream_t s);
+hipError_t pbwt_persist_prepare(int v, size_t lds, int *occ);
+hipError_t launch_pbwt_persist(int v, const TgtDev *tg, BatchDev *B, const PbwtItem *items,
+                               uint32_t grid, size_t lds, hipStream_t s);
 }  // namespace eimp
 
 using namespace eimp;
@@ -75,10 +88,12 @@ inline uint64_t round_up(uint64_t n, uint64_t m) { return (n + m - 1) / m * m; }
 struct DevBuf {
     void *p = nullptr;
     size_t n = 0;
-    int ensure(size_t bytes, const char *what) {
+    // slack: grow by an eighth more than asked, so that slowly growing batches
+    // do not reallocate every call
+    int ensure(size_t bytes, const char *what, bool slack = true) {
         if (bytes <= n && p) return EIMP_OK;
         release();
-        size_t want = std::max<size_t>(bytes + bytes / 8, 256);
+        size_t want = std::max<size_t>(slack ? bytes + bytes / 8 : bytes, 256);
         if (hipMalloc(&p, want) != hipSuccess) {
             p = nullptr;
             (void)hipGetLastError();
@@ -154,6 +169,7 @@ struct eimp_engine {
     std::vector<PendingEvent> pending;
     std::vector<hipEvent_t> event_pool;
     int pbwt_variant = -1;
+    uint64_t panel_gen = 0;  // bumped whenever panel buffers change (stream contexts check it)
     int pbwt_prepared_variant = -1;
     size_t pbwt_prepared_lds = 0;
 
@@ -204,6 +220,7 @@ struct eimp_engine {
         h_stage.release();
     }
     void release_panel() {
+        panel_gen++;
         site.release();
         hap.release();
         cM.release();
@@ -303,7 +320,8 @@ int run_build(eimp_engine *e, uint32_t n, bool digest, bool compact) {
     int v = e->choose_pbwt_variant(Kmax_pad);
     if (v < 0) return v;
     const size_t lds = pbwt_lds_bytes(v, Kmax_pad);
-    if (lds > 160 * 1024) return fail(EIMP_ERR_UNSUPPORTED, "PBWT LDS need exceeds 160 KiB");
+    if (lds > 160 * 1024)
+        return fail(EIMP_ERR_UNSUPPORTED, "K beyond the PBWT kernel's slots or 160 KiB of LDS");
     if (e->pbwt_prepared_variant != v || e->pbwt_prepared_lds < lds) {
         HIPCHK(pbwt_prepare(v, lds));
         e->pbwt_prepared_variant = v;
@@ -346,6 +364,232 @@ int run_build(eimp_engine *e, uint32_t n, bool digest, bool compact) {
                        lds, e->stream));
     e->tend(ti);
     if (digest) HIPCHK(launch_digest_count0(e->d_tg.as<TgtDev>(), n, e->stream));
+    return EIMP_OK;
+}
+
+// Validates target t and fills its input fields and flags.  Ncorr is the
+// engine's (the caller accounts for the skipped sample,
+// src/PBWTPhaser.cpp:629); localK = min(Ncorr, K) (:683).
+int target_inputs(const eimp_engine *e, const eimp_target &x, uint32_t t, float cMmaxSplit,
+                  TgtDev &T) {
+    if (!x.gt) return fail(EIMP_ERR_INVALID, "target " + std::to_string(t) + ": null genotypes");
+    if (x.W < 1 || x.s0 >= e->M || x.W > e->M - x.s0)
+        return fail(EIMP_ERR_INVALID, "target " + std::to_string(t) + ": window outside the panel");
+    if (x.K < 1) return fail(EIMP_ERR_INVALID, "target " + std::to_string(t) + ": K < 1");
+    if (x.ignoreSample >= (int64_t)(e->Nad / 2))
+        return fail(EIMP_ERR_INVALID, "target " + std::to_string(t) + ": ignoreSample out of range");
+    const uint64_t ncorr = e->Ncorr;
+    T.K = (uint32_t)std::min<uint64_t>(x.K, ncorr);
+    T.W = (uint32_t)x.W;
+    T.s0 = x.s0;
+    T.ignore = (int32_t)std::max<int64_t>(x.ignoreSample, -1);
+    T.cMmaxSplit = cMmaxSplit;
+    const bool select = T.K < ncorr;
+    const bool ident = !select && e->Nad == e->Ncorr && x.ignoreSample < 0 && T.K == e->Nad;
+    T.flags = (select ? TF_SELECT : 0u) | (ident ? TF_IDENTITY : 0u);
+    return EIMP_OK;
+}
+
+// Copies the outputs of one target (host record T, device table d_tg) to
+// the caller's buffers; any pointer may be null.
+int copy_target_out(const TgtDev &T, const TgtDev *d_tg, uint32_t t, uint64_t Nad, hipStream_t s,
+                    DevBuf &tmp, uint32_t *besthaps, uint64_t *bhflags, uint32_t *splitsites,
+                    uint64_t *splits, float *cMdiffs, int32_t *count0, uint32_t *pbwt,
+                    uint64_t *cref_rows) {
+    if ((count0 || pbwt || cref_rows) && T.Mp == 0)
+        return fail(EIMP_ERR_STATE, "target " + std::to_string(t) + " was deferred: outputs not built");
+    if (besthaps && T.bh) HIPCHK(hipMemcpy(besthaps, T.bh, (size_t)T.K * 4, hipMemcpyDeviceToHost));
+    if (bhflags && T.bhflags)
+        HIPCHK(hipMemcpy(bhflags, T.bhflags, unit_words(Nad) * 8, hipMemcpyDeviceToHost));
+    if (splitsites && T.S) HIPCHK(hipMemcpy(splitsites, T.splitsites, (size_t)T.S * 4, hipMemcpyDeviceToHost));
+    if (splits && T.splits) HIPCHK(hipMemcpy(splits, T.splits, unit_words(T.W) * 8, hipMemcpyDeviceToHost));
+    if (cMdiffs && T.S && T.cMdiffs)
+        HIPCHK(hipMemcpy(cMdiffs, T.cMdiffs, ((size_t)T.S + 1) * 4, hipMemcpyDeviceToHost));
+    if (count0) HIPCHK(hipMemcpy(count0, T.count0, (size_t)T.Mp * 4, hipMemcpyDeviceToHost));
+    if (pbwt) HIPCHK(hipMemcpy(pbwt, T.out, 2ull * T.Mp * T.RS * 4, hipMemcpyDeviceToHost));
+    if (cref_rows) {
+        const uint32_t rw = (uint32_t)unit_words(T.K);
+        CHK(tmp.ensure((size_t)T.Mp * rw * 8, "cref rows"));
+        HIPCHK(launch_cref_rows(d_tg, t, T.Mp, tmp.as<uint64_t>(), rw, s));
+        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(hipMemcpy(cref_rows, tmp.p, (size_t)T.Mp * rw * 8, hipMemcpyDeviceToHost));
+    }
+    return EIMP_OK;
+}
+
+}  // namespace
+
+
+// ---------------------------------------------------------------------------
+// Stream context: one HIP stream, fixed-capacity buffers, and the batch's
+// launch sequence captured once as a hipGraph.  A submission copies the
+// batch header, the target table and the genotypes (one pinned staging
+// buffer), replays the graph, and copies back the split counts, the table
+// (output placement) and the digests.  Nothing in the sequence depends on
+// host-side sizes: kernels read the batch size from the header and the
+// planner sizes the outputs on the device.
+// ---------------------------------------------------------------------------
+struct eimp_stream {
+    eimp_engine *e = nullptr;
+    hipStream_t stream = nullptr;
+    uint32_t nmax = 0, Wmax = 0, Kmax = 0, flags = 0, max_ww = 0;
+    uint64_t pool_cap = 0, gen = 0;
+    bool timed = false;
+    hipEvent_t kev[EIMP_NKCLASS + 1] = {};  // timed contexts: events between kernel classes
+    double kms[EIMP_NKCLASS] = {};
+    // device buffers; d_in = header (256 B) | TgtDev[nmax] | genotypes
+    DevBuf d_in, d_nerr, d_bh, d_bhflags, d_mono, d_splitsites, d_splits, d_cMdiffs, d_stats,
+        d_digest, d_blockpref, d_pitems, d_pool, d_tmp;
+    uint64_t tab_off = 0, gt_off = 0, gt_cap_words = 0;
+    HostBuf h_in, h_out;  // h_out = stats | digests | TgtDev table
+    // launch configuration
+    int pbwt_v = -1;
+    size_t pbwt_lds = 0;
+    uint32_t cref_grid = 0, pbwt_grid = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // submission state
+    bool pending = false, collected = false;
+    uint32_t n_last = 0;
+
+    BatchDev *d_batch() const { return d_in.as<BatchDev>(); }
+    TgtDev *d_tg() const { return reinterpret_cast<TgtDev *>(d_in.as<char>() + tab_off); }
+    uint32_t *h_stats() const { return h_out.as<uint32_t>(); }
+    uint64_t *h_digest() const { return reinterpret_cast<uint64_t *>(h_out.as<char>() + 16ull * nmax); }
+    TgtDev *h_tg() const { return reinterpret_cast<TgtDev *>(h_out.as<char>() + 32ull * nmax); }
+};
+
+namespace {
+
+int stream_launch_sequence(eimp_stream *s) {
+    eimp_engine *e = s->e;
+    const uint32_t *d_n = &s->d_batch()->n;
+    PlanArgs A{};
+    A.pool = s->d_pool.as<char>();
+    A.pool_cap = s->pool_cap;
+    A.dig_base = s->d_digest.as<uint64_t>();
+    A.blockpref = s->d_blockpref.as<uint32_t>();
+    A.pitems = s->d_pitems.as<PbwtItem>();
+    A.compact = (s->flags & EIMP_BATCH_COMPACT) ? 1u : 0u;
+    A.digest = (s->flags & EIMP_BATCH_DIGEST) ? 1u : 0u;
+    // timed contexts record an event before each kernel class (the planner
+    // counts with the cref class, the count0 digest with the PBWT class)
+    auto mark = [&](int cls) -> int {
+        if (s->timed) HIPCHK(hipEventRecord(s->kev[cls], s->stream));
+        return EIMP_OK;
+    };
+    for (int cls = EIMP_KCLASS_DISTANCE; cls <= EIMP_KCLASS_SPLITS; cls++) {
+        CHK(mark(cls));
+        HIPCHK(launch_prep_dev(cls, e->pd, s->d_tg(), d_n, s->nmax, s->max_ww, s->stream));
+    }
+    CHK(mark(EIMP_KCLASS_CREF));
+    HIPCHK(launch_plan(s->d_tg(), s->d_batch(), A, s->stream));
+    HIPCHK(launch_cref_persist(e->pd, s->d_tg(), s->d_batch(), s->d_blockpref.as<uint32_t>(), s->Kmax,
+                               s->cref_grid, s->stream));
+    CHK(mark(EIMP_KCLASS_PBWT));
+    HIPCHK(launch_pbwt_persist(s->pbwt_v, s->d_tg(), s->d_batch(), s->d_pitems.as<PbwtItem>(),
+                               s->pbwt_grid, s->pbwt_lds, s->stream));
+    if (A.digest) HIPCHK(launch_digest_count0_dev(s->d_tg(), d_n, s->nmax, s->stream));
+    CHK(mark(EIMP_KCLASS_COPY));
+    return EIMP_OK;
+}
+
+void stream_release(eimp_stream *s) {
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->exec) (void)hipGraphExecDestroy(s->exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    for (auto &ev : s->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    for (auto &ev : s->kev)
+        if (ev) (void)hipEventDestroy(ev);
+    DevBuf *bufs[] = {&s->d_in, &s->d_nerr, &s->d_bh, &s->d_bhflags, &s->d_mono, &s->d_splitsites,
+                      &s->d_splits, &s->d_cMdiffs, &s->d_stats, &s->d_digest, &s->d_blockpref,
+                      &s->d_pitems, &s->d_pool, &s->d_tmp};
+    for (auto *b : bufs) b->release();
+    s->h_in.release();
+    s->h_out.release();
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+}
+
+int stream_setup(eimp_stream *s, size_t pool_bytes) {
+    eimp_engine *e = s->e;
+    HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    for (auto &ev : s->ev) HIPCHK(hipEventCreate(&ev));
+    if (s->timed)
+        for (auto &ev : s->kev) HIPCHK(hipEventCreate(&ev));
+    CHK(ensure_hap(e));
+    if (s->Kmax == e->Nad && e->Ncorr == e->Nad) CHK(ensure_pmono(e));
+    s->gen = e->panel_gen;
+    const uint64_t nmax = s->nmax, Nw = unit_words(e->Nad), Ww = unit_words(s->Wmax);
+    s->max_ww = (uint32_t)((s->Wmax + 63) / 64);
+    s->tab_off = 256;
+    s->gt_off = round_up(s->tab_off + nmax * sizeof(TgtDev), 256);
+    s->gt_cap_words = nmax * 2 * Ww;
+    const uint64_t in_bytes = s->gt_off + s->gt_cap_words * 8;
+    CHK(s->d_in.ensure(in_bytes, "stream inputs"));
+    CHK(s->h_in.ensure(in_bytes));
+    std::memset(s->h_in.p, 0, s->gt_off);
+    // any target with K < Ncorr selects (K varies per target up to Kmax)
+    CHK(s->d_nerr.ensure(nmax * round_up(e->Nad, 64) * 4, "distances"));
+    CHK(s->d_bh.ensure(nmax * round_up(s->Kmax, 64) * 4, "best haps"));
+    CHK(s->d_bhflags.ensure(nmax * Nw * 8, "best-hap flags"));
+    CHK(s->d_mono.ensure(nmax * 2 * Ww * 8, "monos"));
+    CHK(s->d_splitsites.ensure(nmax * round_up(s->Wmax + 1, 64) * 4, "split sites"));
+    CHK(s->d_splits.ensure(nmax * Ww * 8, "split bitmap"));
+    CHK(s->d_cMdiffs.ensure(nmax * round_up(s->Wmax + 1, 64) * 4, "cM diffs"));
+    CHK(s->d_stats.ensure(nmax * 16, "stats"));
+    CHK(s->d_digest.ensure(nmax * 16, "digests"));
+    CHK(s->d_blockpref.ensure((nmax + 1) * 4, "block prefix"));
+    CHK(s->d_pitems.ensure(2 * nmax * sizeof(PbwtItem), "pbwt items"));
+    CHK(s->d_pool.ensure(pool_bytes, "output pool", false));
+    s->pool_cap = pool_bytes;
+    CHK(s->h_out.ensure(32 * nmax + nmax * sizeof(TgtDev)));
+    // kernels and grids
+    const uint32_t Kpad = (uint32_t)round_up(s->Kmax, 64);
+    if (s->Kmax > cref_max_k()) return fail(EIMP_ERR_UNSUPPORTED, "K beyond the cref kernel limit");
+    if (cref_lds_bytes(e->pd) > 160 * 1024)
+        return fail(EIMP_ERR_UNSUPPORTED, "panel too wide for the cref kernel's LDS staging");
+    int v = e->choose_pbwt_variant(Kpad);
+    if (v < 0) return v;
+    const size_t lds = pbwt_lds_bytes(v, Kpad);
+    if (lds > 160 * 1024)
+        return fail(EIMP_ERR_UNSUPPORTED, "K beyond the PBWT kernel's slots or 160 KiB of LDS");
+    s->pbwt_v = v;
+    s->pbwt_lds = lds;
+    int ncu = 0, occ_c = 0, occ_p = 0;
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, e->dev));
+    HIPCHK(cref_persist_prepare(e->pd, s->Kmax, &occ_c));
+    HIPCHK(pbwt_persist_prepare(v, lds, &occ_p));
+    s->cref_grid = (uint32_t)(std::max(ncu, 1) * std::max(occ_c, 1));
+    s->pbwt_grid = (uint32_t)std::min<uint64_t>(2 * nmax, (uint64_t)std::max(ncu, 1) * std::max(occ_p, 1));
+    // fixed per-slot pointers of the target table
+    TgtDev *tab = reinterpret_cast<TgtDev *>(s->h_in.as<char>() + s->tab_off);
+    for (uint64_t t = 0; t < nmax; t++) {
+        TgtDev &T = tab[t];
+        T.nerr = s->d_nerr.as<uint32_t>() + t * round_up(e->Nad, 64);
+        T.bh = s->d_bh.as<uint32_t>() + t * round_up(s->Kmax, 64);
+        T.bhflags = s->d_bhflags.as<uint64_t>() + t * Nw;
+        T.mand_w = s->d_mono.as<uint64_t>() + t * 2 * Ww;
+        T.mor_w = T.mand_w + Ww;
+        T.splitsites = s->d_splitsites.as<uint32_t>() + t * round_up(s->Wmax + 1, 64);
+        T.splits = s->d_splits.as<uint64_t>() + t * Ww;
+        T.cMdiffs = s->d_cMdiffs.as<float>() + t * round_up(s->Wmax + 1, 64);
+        T.stats = s->d_stats.as<uint32_t>() + 4 * t;
+    }
+    if (!(s->flags & (EIMP_STREAM_NO_GRAPH | EIMP_STREAM_TIMED))) {
+        HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = stream_launch_sequence(s);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(s->stream, &g);
+        if (rc != EIMP_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ec != hipSuccess) return fail(EIMP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+        s->graph = g;
+        HIPCHK(hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0));
+    }
     return EIMP_OK;
 }
 
@@ -422,6 +666,7 @@ int eimp_engine_initialize(eimp_engine *e, size_t K, size_t Kwords, size_t M, si
         (void)hipStreamSynchronize(e->stream);
         e->release_panel();
     }
+    e->panel_gen++;
     e->K = K;
     e->Kwords = Kwords;
     e->M = M;
@@ -459,6 +704,7 @@ int eimp_engine_set_panel_extras(eimp_engine *e, const float *cM, const uint64_t
                                  size_t hapStride, const uint8_t *haploidRef) {
     if (!e || !e->initialized) return fail(EIMP_ERR_STATE, "engine not initialized");
     CHK(set_device(e));
+    e->panel_gen++;
     if (cM) {
         CHK(e->cM.ensure(e->M * sizeof(float), "cM"));
         HIPCHK(hipMemcpy(e->cM.p, cM, e->M * sizeof(float), hipMemcpyHostToDevice));
@@ -504,24 +750,10 @@ int eimp_engine_run_batch(eimp_engine *e, size_t n, const eimp_target *targets, 
     e->h_tg.assign(nt, TgtDev{});
     for (uint32_t t = 0; t < nt; t++) {
         const eimp_target &x = targets[t];
-        if (!x.gt) return fail(EIMP_ERR_INVALID, "target " + std::to_string(t) + ": null genotypes");
-        if (x.W < 1 || x.s0 + x.W > e->M)
-            return fail(EIMP_ERR_INVALID, "target " + std::to_string(t) + ": window outside the panel");
-        if (x.K < 1) return fail(EIMP_ERR_INVALID, "target " + std::to_string(t) + ": K < 1");
-        if (x.ignoreSample >= (int64_t)(e->Nad / 2))
-            return fail(EIMP_ERR_INVALID, "target " + std::to_string(t) + ": ignoreSample out of range");
         TgtDev &T = e->h_tg[t];
-        // Ncorr is the engine's (the caller accounts for the skipped sample,
-        // src/PBWTPhaser.cpp:629); localK = min(Ncorr, K) (:683)
-        const uint64_t ncorr = e->Ncorr;
-        T.K = (uint32_t)std::min<uint64_t>(x.K, ncorr);
-        T.W = (uint32_t)x.W;
-        T.s0 = x.s0;
-        T.ignore = (int32_t)x.ignoreSample;
-        T.cMmaxSplit = cMmaxSplit;
-        const bool select = T.K < ncorr;
-        const bool ident = !select && e->Nad == e->Ncorr && x.ignoreSample < 0 && T.K == e->Nad;
-        T.flags = (select ? TF_SELECT : 0u) | (ident ? TF_IDENTITY : 0u);
+        CHK(target_inputs(e, x, t, cMmaxSplit, T));
+        const bool select = (T.flags & TF_SELECT) != 0;
+        const bool ident = (T.flags & TF_IDENTITY) != 0;
         gt_off[t] = gt_words;
         gt_words += 2 * unit_words(x.W);
         ww_off[t] = w_words;
@@ -731,25 +963,8 @@ int eimp_engine_copy_target(eimp_engine *e, size_t t, uint32_t *besthaps, uint64
     if (!e || !e->initialized) return fail(EIMP_ERR_STATE, "engine not initialized");
     if (t >= e->last_n || t >= e->h_tg.size()) return fail(EIMP_ERR_INVALID, "target index out of range");
     CHK(set_device(e));
-    const TgtDev &T = e->h_tg[t];
-    if (besthaps && T.bh) HIPCHK(hipMemcpy(besthaps, T.bh, (size_t)T.K * 4, hipMemcpyDeviceToHost));
-    if (bhflags && T.bhflags)
-        HIPCHK(hipMemcpy(bhflags, T.bhflags, unit_words(e->Nad) * 8, hipMemcpyDeviceToHost));
-    if (splitsites && T.S) HIPCHK(hipMemcpy(splitsites, T.splitsites, (size_t)T.S * 4, hipMemcpyDeviceToHost));
-    if (splits && T.splits) HIPCHK(hipMemcpy(splits, T.splits, unit_words(T.W) * 8, hipMemcpyDeviceToHost));
-    if (cMdiffs && T.S && T.cMdiffs)
-        HIPCHK(hipMemcpy(cMdiffs, T.cMdiffs, ((size_t)T.S + 1) * 4, hipMemcpyDeviceToHost));
-    if (count0) HIPCHK(hipMemcpy(count0, T.count0, (size_t)T.Mp * 4, hipMemcpyDeviceToHost));
-    if (pbwt) HIPCHK(hipMemcpy(pbwt, T.out, 2ull * T.Mp * T.RS * 4, hipMemcpyDeviceToHost));
-    if (cref_rows) {
-        const uint32_t rw = (uint32_t)unit_words(T.K);
-        CHK(e->d_tmp.ensure((size_t)T.Mp * rw * 8, "cref rows"));
-        HIPCHK(launch_cref_rows(e->d_tg.as<TgtDev>(), (uint32_t)t, T.Mp, e->d_tmp.as<uint64_t>(), rw,
-                                e->stream));
-        HIPCHK(hipStreamSynchronize(e->stream));
-        HIPCHK(hipMemcpy(cref_rows, e->d_tmp.p, (size_t)T.Mp * rw * 8, hipMemcpyDeviceToHost));
-    }
-    return EIMP_OK;
+    return copy_target_out(e->h_tg[t], e->d_tg.as<TgtDev>(), (uint32_t)t, e->Nad, e->stream, e->d_tmp,
+                           besthaps, bhflags, splitsites, splits, cMdiffs, count0, pbwt, cref_rows);
 }
 
 int eimp_engine_set_timing(eimp_engine *e, int enable) {
@@ -790,6 +1005,204 @@ int eimp_pbwt_variant_info(int v, int *threads, int *cpw, int *max_k) {
     if (threads) *threads = pbwt_variant_threads(v);
     if (cpw) *cpw = pbwt_variant_cpw(v);
     if (max_k) *max_k = (int)pbwt_variant_maxk(v);
+    return EIMP_OK;
+}
+
+
+int eimp_engine_mem_info(eimp_engine *e, size_t *free_bytes, size_t *total_bytes) {
+    if (!e) return fail(EIMP_ERR_INVALID, "null engine");
+    CHK(set_device(e));
+    size_t f = 0, t = 0;
+    HIPCHK(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return EIMP_OK;
+}
+
+int eimp_stream_create(eimp_engine *e, size_t max_targets, size_t max_window, size_t max_K,
+                       size_t pool_bytes, uint32_t flags, eimp_stream **out) {
+    if (!out) return fail(EIMP_ERR_INVALID, "null stream pointer");
+    *out = nullptr;
+    if (!e || !e->initialized) return fail(EIMP_ERR_STATE, "engine not initialized");
+    if (!e->have_cM) return fail(EIMP_ERR_STATE, "genetic positions not set (eimp_engine_set_panel_extras)");
+    if (max_targets < 1 || max_targets > 1024)
+        return fail(EIMP_ERR_INVALID, "max_targets must be in [1, 1024]");
+    if (max_window < 1 || max_window > e->M) return fail(EIMP_ERR_INVALID, "max_window must be in [1, M]");
+    if (max_K < 1) return fail(EIMP_ERR_INVALID, "max_K < 1");
+    if (pool_bytes < 256) return fail(EIMP_ERR_INVALID, "pool_bytes too small");
+    if (flags & ~(EIMP_BATCH_DIGEST | EIMP_BATCH_COMPACT | EIMP_STREAM_NO_GRAPH | EIMP_STREAM_TIMED))
+        return fail(EIMP_ERR_INVALID, "unknown stream flags");
+    CHK(set_device(e));
+    eimp_stream *s = new eimp_stream();
+    s->e = e;
+    s->nmax = (uint32_t)max_targets;
+    s->Wmax = (uint32_t)max_window;
+    s->Kmax = (uint32_t)std::min<uint64_t>(max_K, e->Ncorr);
+    s->flags = flags;
+    s->timed = (flags & EIMP_STREAM_TIMED) != 0;
+    const int rc = stream_setup(s, pool_bytes);
+    if (rc != EIMP_OK) {
+        const std::string msg = g_err;
+        stream_release(s);
+        delete s;
+        (void)hipGetLastError();
+        return fail(rc, msg);
+    }
+    *out = s;
+    return EIMP_OK;
+}
+
+int eimp_stream_submit(eimp_stream *s, size_t n, const eimp_target *targets, float cMmaxSplit) {
+    if (!s) return fail(EIMP_ERR_INVALID, "null stream");
+    if (s->pending) return fail(EIMP_ERR_STATE, "previous submission not collected (eimp_stream_wait)");
+    eimp_engine *e = s->e;
+    if (s->gen != e->panel_gen || !e->initialized)
+        return fail(EIMP_ERR_STATE, "panel changed since the context was created");
+    if (n > s->nmax) return fail(EIMP_ERR_INVALID, "more targets than the context's max_targets");
+    if (n && !targets) return fail(EIMP_ERR_INVALID, "null targets");
+    CHK(set_device(e));
+    const uint32_t nt = (uint32_t)n;
+    BatchDev *hb = s->h_in.as<BatchDev>();
+    *hb = BatchDev{};
+    hb->n = nt;
+    TgtDev *tab = reinterpret_cast<TgtDev *>(s->h_in.as<char>() + s->tab_off);
+    uint64_t *gtst = reinterpret_cast<uint64_t *>(s->h_in.as<char>() + s->gt_off);
+    uint64_t *gtdev = reinterpret_cast<uint64_t *>(s->d_in.as<char>() + s->gt_off);
+    uint64_t gw = 0;
+    for (uint32_t t = 0; t < nt; t++) {
+        const eimp_target &x = targets[t];
+        TgtDev &T = tab[t];
+        CHK(target_inputs(e, x, t, cMmaxSplit, T));
+        if (x.W > s->Wmax) return fail(EIMP_ERR_INVALID, "target " + std::to_string(t) + ": window beyond max_window");
+        if (T.K > s->Kmax) return fail(EIMP_ERR_INVALID, "target " + std::to_string(t) + ": K beyond max_K");
+        const uint64_t w = 2 * unit_words(x.W);
+        std::memcpy(gtst + gw, x.gt, w * 8);
+        T.gt = gtdev + gw;
+        gw += w;
+        if (T.flags & TF_IDENTITY) {
+            T.mand = e->pand.as<uint64_t>();
+            T.mor = e->por.as<uint64_t>();
+            T.mono_off = T.s0;
+        } else {
+            T.mand = T.mand_w;
+            T.mor = T.mor_w;
+            T.mono_off = 0;
+        }
+        T.cref = nullptr;
+        T.count0 = nullptr;
+        T.out = nullptr;
+        T.digest = nullptr;
+        T.Mp = T.Kpad = T.RS = T.S = 0;
+    }
+    HIPCHK(hipEventRecord(s->ev[0], s->stream));
+    HIPCHK(hipMemcpyAsync(s->d_in.p, s->h_in.p, s->tab_off + (size_t)nt * sizeof(TgtDev),
+                          hipMemcpyHostToDevice, s->stream));
+    if (gw) HIPCHK(hipMemcpyAsync(gtdev, gtst, gw * 8, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipEventRecord(s->ev[1], s->stream));
+    if (nt) {
+        if (s->exec) HIPCHK(hipGraphLaunch(s->exec, s->stream));
+        else CHK(stream_launch_sequence(s));
+    }
+    HIPCHK(hipEventRecord(s->ev[2], s->stream));
+    if (nt) {
+        HIPCHK(hipMemcpyAsync(s->h_stats(), s->d_stats.p, (size_t)nt * 16, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipMemcpyAsync(s->h_tg(), s->d_tg(), (size_t)nt * sizeof(TgtDev), hipMemcpyDeviceToHost,
+                              s->stream));
+        if (s->flags & EIMP_BATCH_DIGEST)
+            HIPCHK(hipMemcpyAsync(s->h_digest(), s->d_digest.p, (size_t)nt * 16, hipMemcpyDeviceToHost,
+                                  s->stream));
+    }
+    HIPCHK(hipEventRecord(s->ev[3], s->stream));
+    s->pending = true;
+    s->collected = false;
+    s->n_last = nt;
+    return EIMP_OK;
+}
+
+int eimp_stream_query(eimp_stream *s) {
+    if (!s) return fail(EIMP_ERR_INVALID, "null stream");
+    if (!s->pending) return EIMP_OK;
+    CHK(set_device(s->e));
+    const hipError_t q = hipEventQuery(s->ev[3]);
+    if (q == hipSuccess) return EIMP_OK;
+    if (q == hipErrorNotReady) return EIMP_BUSY;
+    return fail(EIMP_ERR_HIP, std::string("hipEventQuery: ") + hipGetErrorString(q));
+}
+
+int eimp_stream_wait(eimp_stream *s, eimp_target_result *results, size_t *ndeferred, double *times) {
+    if (!s) return fail(EIMP_ERR_INVALID, "null stream");
+    if (!s->pending) return fail(EIMP_ERR_STATE, "no submission to wait for");
+    CHK(set_device(s->e));
+    HIPCHK(hipEventSynchronize(s->ev[3]));
+    s->pending = false;
+    s->collected = true;
+    const uint32_t nt = s->n_last;
+    size_t nd = 0;
+    const bool digest = (s->flags & EIMP_BATCH_DIGEST) != 0;
+    for (uint32_t t = 0; t < nt; t++) {
+        TgtDev &T = s->h_tg()[t];
+        const uint32_t *st = s->h_stats() + 4 * t;
+        const bool deferred = st[2] != 0;
+        if (deferred) {
+            T.Mp = 0;
+            nd++;
+        }
+        if (results) {
+            results[t].S = st[0];
+            results[t].Mp = deferred ? 0 : T.Mp;
+            results[t].ncalls = st[1];
+            results[t].K = T.K;
+            results[t].digest_pbwt = digest && !deferred ? s->h_digest()[2 * t] : 0;
+            results[t].digest_count0 = digest && !deferred ? s->h_digest()[2 * t + 1] : 0;
+        }
+    }
+    if (ndeferred) *ndeferred = nd;
+    if (times) {
+        float a = 0, b = 0;
+        HIPCHK(hipEventElapsedTime(&a, s->ev[1], s->ev[2]));
+        HIPCHK(hipEventElapsedTime(&b, s->ev[0], s->ev[3]));
+        times[0] = a;
+        times[1] = b;
+    }
+    for (int c = 0; c < EIMP_NKCLASS; c++) s->kms[c] = 0;
+    if (s->timed && nt) {
+        for (int c = EIMP_KCLASS_DISTANCE; c < EIMP_KCLASS_COPY; c++) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, s->kev[c], s->kev[c + 1]));
+            s->kms[c] = ms;
+        }
+        float ms = 0;  // copies: header/table/genotypes in, results out
+        HIPCHK(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
+        s->kms[EIMP_KCLASS_COPY] = ms;
+        HIPCHK(hipEventElapsedTime(&ms, s->ev[2], s->ev[3]));
+        s->kms[EIMP_KCLASS_COPY] += ms;
+    }
+    return EIMP_OK;
+}
+
+int eimp_stream_kernel_ms(eimp_stream *s, double *ms) {
+    if (!s || !ms) return fail(EIMP_ERR_INVALID, "null argument");
+    if (!s->timed) return fail(EIMP_ERR_STATE, "context not created with EIMP_STREAM_TIMED");
+    for (int c = 0; c < EIMP_NKCLASS; c++) ms[c] = s->kms[c];
+    return EIMP_OK;
+}
+
+int eimp_stream_copy_target(eimp_stream *s, size_t t, uint32_t *besthaps, uint64_t *bhflags,
+                            uint32_t *splitsites, uint64_t *splits, float *cMdiffs, int32_t *count0,
+                            uint32_t *pbwt, uint64_t *cref_rows) {
+    if (!s) return fail(EIMP_ERR_INVALID, "null stream");
+    if (s->pending || !s->collected) return fail(EIMP_ERR_STATE, "no collected submission");
+    if (t >= s->n_last) return fail(EIMP_ERR_INVALID, "target index out of range");
+    CHK(set_device(s->e));
+    return copy_target_out(s->h_tg()[t], s->d_tg(), (uint32_t)t, s->e->Nad, s->stream, s->d_tmp,
+                           besthaps, bhflags, splitsites, splits, cMdiffs, count0, pbwt, cref_rows);
+}
+
+int eimp_stream_destroy(eimp_stream *s) {
+    if (!s) return EIMP_OK;
+    (void)set_device(s->e);
+    stream_release(s);
+    delete s;
     return EIMP_OK;
 }
 

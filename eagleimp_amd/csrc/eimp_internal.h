@@ -47,7 +47,7 @@ struct TgtDev {
     uint32_t *splitsites;  // W
     uint64_t *splits;      // unit_words(W)
     float *cMdiffs;        // W + 1
-    uint32_t *stats;       // [0] S, [1] ncalls, [2] first het, [3] reserved
+    uint32_t *stats;       // [0] S, [1] ncalls, [2] deferred (stream contexts), [3] best haps set
     // phase B (build)
     uint16_t *cref;        // nblk x Kpad, transposed 16-row blocks
     int32_t *count0;       // Mp
@@ -59,6 +59,18 @@ struct TgtDev {
     uint32_t S;
 };
 
+// Device-side header of a batch run by a stream context (device-driven
+// path: no host synchronisation between the phases).
+struct BatchDev {
+    uint32_t n;         // targets in this submission (written by the host)
+    uint32_t ncref;     // cref blocks queued by k_plan
+    uint32_t npbwt;     // PBWT items queued by k_plan
+    uint32_t nfit;      // targets whose outputs fit the pools
+    uint32_t cref_ctr;  // work-queue counters (zeroed by k_plan)
+    uint32_t pbwt_ctr;
+    uint32_t pad[2];
+};
+
 // cref work item: target t, blocks [b0, b1)
 struct CrefItem {
     uint32_t t, b0, b1, pad;
@@ -68,6 +80,30 @@ struct CrefItem {
 struct PbwtItem {
     uint32_t t, dir;
 };
+
+// Output pool and queues of a stream context, for k_plan.  Each target gets
+// one contiguous region of the pool: cref blocks, count0, PBWT rows.
+struct PlanArgs {
+    char *pool;
+    uint64_t pool_cap;
+    uint64_t *dig_base;    // 2 words per target
+    uint32_t *blockpref;   // nmax + 1 prefix sums of cref blocks
+    PbwtItem *pitems;      // 2 * nmax
+    uint32_t compact;      // rows of 2*ceil(K/32) instead of 4*Kwords
+    uint32_t digest;
+};
+
+// Bytes of a target's region in a stream context's pool (each part rounded
+// to 256 bytes): cref blocks, count0, forward + backward PBWT rows.
+__host__ __device__ inline uint64_t region_cref_bytes(uint32_t Mp, uint32_t Kpad) {
+    return ((uint64_t)((Mp + 15) / 16) * Kpad * 2 + 255) / 256 * 256;
+}
+__host__ __device__ inline uint64_t region_c0_bytes(uint32_t Mp) {
+    return ((uint64_t)Mp * 4 + 255) / 256 * 256;
+}
+__host__ __device__ inline uint64_t region_out_bytes(uint32_t Mp, uint32_t RS) {
+    return (2ull * Mp * RS * 4 + 255) / 256 * 256;
+}
 
 // Panel in device memory (resident)
 struct PanelDev {

@@ -18,6 +18,8 @@
 //                    (Target::condensedReferenceT, src/Target.cpp:953-1059)
 //   k_pbwt           compressed forward / backward PBWT
 //                    (PBWT::buildComprPBWT, src/PBWT.cpp:207-285)
+#include <map>
+#include <mutex>
 #include <type_traits>
 
 #include "eimp_internal.h"
@@ -121,8 +123,12 @@ __global__ __launch_bounds__(256) void k_panel_mono(PanelDev P, uint64_t *__rest
 // a3: distances.  grid (ceil(N_ad/16), nsel), block 256, a wave per haplotype
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_distance(PanelDev P, const TgtDev *__restrict__ tg,
-                                                 const uint32_t *__restrict__ tlist) {
-    const TgtDev &T = tg[tlist[blockIdx.y]];
+                                                 const uint32_t *__restrict__ tlist,
+                                                 const uint32_t *__restrict__ d_n) {
+    // tlist: targets that select; without it every target of the batch (size *d_n)
+    if (!tlist && blockIdx.y >= *d_n) return;
+    const TgtDev &T = tg[tlist ? tlist[blockIdx.y] : blockIdx.y];
+    if (!(T.flags & TF_SELECT)) return;
     const uint32_t wave = threadIdx.x >> 6, l = lane_id();
     const uint64_t Ww = (T.W + 63) / 64;
     const uint64_t *__restrict__ gt = T.gt;
@@ -169,7 +175,9 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *s_w, u
     return before + inc - v;
 }
 
-__global__ __launch_bounds__(SEL_NT) void k_select(PanelDev P, const TgtDev *__restrict__ tg) {
+__global__ __launch_bounds__(SEL_NT) void k_select(PanelDev P, const TgtDev *__restrict__ tg,
+                                                   const uint32_t *__restrict__ d_n) {
+    if (d_n && blockIdx.x >= *d_n) return;
     const TgtDev &T = tg[blockIdx.x];
     __shared__ uint32_t hist[1024];
     __shared__ uint32_t s_w[SEL_NW];
@@ -257,8 +265,12 @@ __global__ __launch_bounds__(SEL_NT) void k_select(PanelDev P, const TgtDev *__r
 // 1024: 16 waves split the K chosen rows, lanes cover 64 window words.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void k_monos(PanelDev P, const TgtDev *__restrict__ tg,
-                                               const uint32_t *__restrict__ tlist) {
-    const TgtDev &T = tg[tlist[blockIdx.y]];
+                                               const uint32_t *__restrict__ tlist,
+                                               const uint32_t *__restrict__ d_n) {
+    // tlist: targets that need monos; without it every target of the batch (size *d_n)
+    if (!tlist && blockIdx.y >= *d_n) return;
+    const TgtDev &T = tg[tlist ? tlist[blockIdx.y] : blockIdx.y];
+    if (T.flags & TF_IDENTITY) return;
     __shared__ uint64_t s_and[16][64], s_or[16][64];
     const uint32_t wave = threadIdx.x >> 6, l = lane_id();
     const uint64_t Ww = (T.W + 63) / 64;
@@ -301,7 +313,9 @@ __global__ __launch_bounds__(1024) void k_monos(PanelDev P, const TgtDev *__rest
 // splits and hom candidates are resolved lowest-first against the cM of the
 // previous split (see the comment in the loop).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_splits(PanelDev P, const TgtDev *__restrict__ tg) {
+__global__ __launch_bounds__(64) void k_splits(PanelDev P, const TgtDev *__restrict__ tg,
+                                              const uint32_t *__restrict__ d_n) {
+    if (d_n && blockIdx.x >= *d_n) return;
     const TgtDev &T = tg[blockIdx.x];
     const uint32_t l = lane_id();
     const uint32_t W = T.W;
@@ -405,7 +419,9 @@ __global__ __launch_bounds__(64) void k_splits(PanelDev P, const TgtDev *__restr
 // Words skipped before the first het must be zero in the split bitmap: the
 // loop `continue`s there without writing.  This pass clears them (grid n,
 // block 64) -- it runs before k_splits.
-__global__ __launch_bounds__(64) void k_splits_clear(const TgtDev *__restrict__ tg) {
+__global__ __launch_bounds__(64) void k_splits_clear(const TgtDev *__restrict__ tg,
+                                                    const uint32_t *__restrict__ d_n) {
+    if (d_n && blockIdx.x >= *d_n) return;
     const TgtDev &T = tg[blockIdx.x];
     const uint32_t nwords = ((T.W + 511) / 512) * 8;
     for (uint32_t w = lane_id(); w < nwords; w += 64) T.splits[w] = 0ull;
@@ -502,156 +518,165 @@ __device__ __forceinline__ void cref_fill(const PanelDev &P, const TgtDev &T, ui
     }
 }
 
-// K chosen haplotypes by index (any target).  EPT haplotypes per thread.
+// Block b of a target, K chosen haplotypes by index (any target).  EPT
+// haplotypes per thread.
+template <int EPT>
+__device__ __forceinline__ void cref_block_gather(const PanelDev &P, const TgtDev &T, uint32_t b,
+                                                  uint64_t *srow, uint32_t *cnt) {
+    const uint32_t tid = threadIdx.x, l = lane_id();
+    const uint32_t K = T.K, S = T.S, Kpad = T.Kpad, Mp = T.Mp;
+    const uint32_t nw = (P.N_ad + 63) / 64;
+    const uint32_t G = cref_group(nw);
+    const bool ident = (T.flags & TF_IDENTITY) != 0;
+    uint32_t idx[EPT], V[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+        const uint32_t k = tid + CREF_NT * e;
+        idx[e] = k < K ? (ident ? k : T.bh[k]) : 0u;
+        V[e] = 0;
+    }
+    if (tid < 16) cnt[tid] = 0;
+    const uint32_t send = min(8 * b + 8, S + 1);
+    for (uint32_t sa = 8 * b; sa < send; sa += G) {
+        const uint32_t sb = min(sa + G, send);
+        cref_fill(P, T, sa, sb, nw, srow);
+        __syncthreads();
+        const uint32_t nrows = 2 * (sb - sa) - (sb == S + 1 ? 1u : 0u);
+        for (uint32_t rr = 0; rr < nrows; rr++) {
+            const uint32_t rb = 2 * (sa - 8 * b) + rr;
+            const uint64_t *row = srow + (uint64_t)rr * nw;
+            uint32_t ones = 0;
+#pragma unroll
+            for (int e = 0; e < EPT; e++) {
+                const uint32_t k = tid + CREF_NT * e;
+                const uint32_t bit = k < K ? (uint32_t)((row[idx[e] >> 6] >> (idx[e] & 63)) & 1ull) : 0u;
+                V[e] |= bit << rb;
+                ones += (uint32_t)__popcll(__ballot(bit));
+            }
+            if (l == 0) atomicAdd(&cnt[rb], ones);
+        }
+        __syncthreads();
+    }
+    uint16_t *__restrict__ dst = T.cref + (uint64_t)b * Kpad;
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+        const uint32_t k = tid + CREF_NT * e;
+        if (k < Kpad) dst[k] = k < K ? (uint16_t)V[e] : (uint16_t)0xFFFFu;
+    }
+    if (tid < 16) {
+        const uint32_t r = 16 * b + tid;
+        if (r < Mp) T.count0[r] = (int32_t)K - (int32_t)cnt[tid];
+    }
+    __syncthreads();
+}
+
+// Block b of a target taking all N haplotypes in panel order (TF_IDENTITY):
+// no gather.  Each thread owns GPT groups of 64 haplotypes, reads their word of
+// every staged row and transposes into 64 uint16 values, written as 128
+// contiguous bytes.
+template <int GPT>
+__device__ __forceinline__ void cref_block_ident(const PanelDev &P, const TgtDev &T, uint32_t b,
+                                                 uint64_t *srow, uint32_t *cnt) {
+    const uint32_t tid = threadIdx.x, l = lane_id();
+    const uint32_t K = T.K, S = T.S, Kpad = T.Kpad, Mp = T.Mp;
+    const uint32_t nw = (P.N_ad + 63) / 64;  // == Kpad / 64 (K == N_ad)
+    const uint32_t G = cref_group(nw);
+    uint32_t V[GPT][32];  // two uint16 values per register
+#pragma unroll
+    for (int e = 0; e < GPT; e++)
+#pragma unroll
+        for (int i = 0; i < 32; i++) V[e][i] = 0;
+    if (tid < 16) cnt[tid] = 0;
+    const uint32_t send = min(8 * b + 8, S + 1);
+    for (uint32_t sa = 8 * b; sa < send; sa += G) {
+        const uint32_t sb = min(sa + G, send);
+        cref_fill(P, T, sa, sb, nw, srow);
+        __syncthreads();
+        const uint32_t nrows = 2 * (sb - sa) - (sb == S + 1 ? 1u : 0u);
+        for (uint32_t rr = 0; rr < nrows; rr++) {
+            const uint32_t rb = 2 * (sa - 8 * b) + rr;
+            const uint64_t *row = srow + (uint64_t)rr * nw;
+            uint32_t ones = 0;
+#pragma unroll
+            for (int e = 0; e < GPT; e++) {
+                const uint32_t q = tid + CREF_NT * e;
+                const uint64_t wv = q < nw ? row[q] : 0ull;
+                ones += (uint32_t)__popcll(wv);
+                const uint32_t lo = (uint32_t)wv, hi = (uint32_t)(wv >> 32);
+#pragma unroll
+                for (int i = 0; i < 16; i++) {
+                    // haplotypes 2i, 2i+1 (low word) and 32+2i, 33+2i (high word)
+                    V[e][i] |= (((lo >> (2 * i)) & 1u) << rb) | (((lo >> (2 * i + 1)) & 1u) << (16 + rb));
+                    V[e][16 + i] |= (((hi >> (2 * i)) & 1u) << rb) | (((hi >> (2 * i + 1)) & 1u) << (16 + rb));
+                }
+            }
+            ones = wave_sum(ones);
+            if (l == 0) atomicAdd(&cnt[rb], ones);
+        }
+        __syncthreads();
+    }
+    uint16_t *__restrict__ dst = T.cref + (uint64_t)b * Kpad;
+#pragma unroll
+    for (int e = 0; e < GPT; e++) {
+        const uint32_t q = tid + CREF_NT * e;
+        if (q < nw) {
+            if (64 * q + 64 > K) {  // last group: haplotypes >= K are padding
+#pragma unroll
+                for (int i = 0; i < 32; i++) {
+                    const uint32_t k0 = 64 * q + 2 * i;
+                    if (k0 >= K) V[e][i] |= 0xFFFFu;
+                    if (k0 + 1 >= K) V[e][i] |= 0xFFFF0000u;
+                }
+            }
+            uint4 *d4 = (uint4 *)(dst + 64 * q);
+#pragma unroll
+            for (int i = 0; i < 8; i++)
+                d4[i] = make_uint4(V[e][4 * i], V[e][4 * i + 1], V[e][4 * i + 2], V[e][4 * i + 3]);
+        }
+    }
+    if (tid < 16) {
+        const uint32_t r = 16 * b + tid;
+        if (r < Mp) T.count0[r] = (int32_t)K - (int32_t)cnt[tid];
+    }
+    __syncthreads();
+}
+
 template <int EPT>
 __global__ __launch_bounds__(CREF_NT) void k_cref(PanelDev P, const TgtDev *__restrict__ tg,
                                                  const CrefItem *__restrict__ items) {
     extern __shared__ uint64_t srow[];
     __shared__ uint32_t cnt[16];
     const CrefItem it = items[blockIdx.x];
-    const TgtDev &T = tg[it.t];
-    const uint32_t tid = threadIdx.x, l = lane_id();
-    const uint32_t K = T.K, S = T.S, Kpad = T.Kpad, Mp = T.Mp;
-    const uint32_t nw = (P.N_ad + 63) / 64;
-    const uint32_t G = cref_group(nw);
-    const bool ident = (T.flags & TF_IDENTITY) != 0;
-    uint32_t idx[EPT];
-#pragma unroll
-    for (int e = 0; e < EPT; e++) {
-        const uint32_t k = tid + CREF_NT * e;
-        idx[e] = k < K ? (ident ? k : T.bh[k]) : 0u;
-    }
-    for (uint32_t b = it.b0; b < it.b1; b++) {
-        uint32_t V[EPT];
-#pragma unroll
-        for (int e = 0; e < EPT; e++) V[e] = 0;
-        if (tid < 16) cnt[tid] = 0;
-        const uint32_t send = min(8 * b + 8, S + 1);
-        for (uint32_t sa = 8 * b; sa < send; sa += G) {
-            const uint32_t sb = min(sa + G, send);
-            cref_fill(P, T, sa, sb, nw, srow);
-            __syncthreads();
-            const uint32_t nrows = 2 * (sb - sa) - (sb == S + 1 ? 1u : 0u);
-            for (uint32_t rr = 0; rr < nrows; rr++) {
-                const uint32_t rb = 2 * (sa - 8 * b) + rr;
-                const uint64_t *row = srow + (uint64_t)rr * nw;
-                uint32_t ones = 0;
-#pragma unroll
-                for (int e = 0; e < EPT; e++) {
-                    const uint32_t k = tid + CREF_NT * e;
-                    const uint32_t bit = k < K ? (uint32_t)((row[idx[e] >> 6] >> (idx[e] & 63)) & 1ull) : 0u;
-                    V[e] |= bit << rb;
-                    ones += (uint32_t)__popcll(__ballot(bit));
-                }
-                if (l == 0) atomicAdd(&cnt[rb], ones);
-            }
-            __syncthreads();
-        }
-        uint16_t *__restrict__ dst = T.cref + (uint64_t)b * Kpad;
-#pragma unroll
-        for (int e = 0; e < EPT; e++) {
-            const uint32_t k = tid + CREF_NT * e;
-            if (k < Kpad) dst[k] = k < K ? (uint16_t)V[e] : (uint16_t)0xFFFFu;
-        }
-        if (tid < 16) {
-            const uint32_t r = 16 * b + tid;
-            if (r < Mp) T.count0[r] = (int32_t)K - (int32_t)cnt[tid];
-        }
-        __syncthreads();
-    }
+    for (uint32_t b = it.b0; b < it.b1; b++) cref_block_gather<EPT>(P, tg[it.t], b, srow, cnt);
 }
 
-// All N haplotypes in panel order (TF_IDENTITY): no gather.  Each thread
-// owns GPT groups of 64 haplotypes, reads their word of every staged row and
-// transposes into 64 uint16 values, written as 128 contiguous bytes.
 template <int GPT>
 __global__ __launch_bounds__(CREF_NT) void k_cref_ident(PanelDev P, const TgtDev *__restrict__ tg,
                                                        const CrefItem *__restrict__ items) {
     extern __shared__ uint64_t srow[];
     __shared__ uint32_t cnt[16];
     const CrefItem it = items[blockIdx.x];
-    const TgtDev &T = tg[it.t];
-    const uint32_t tid = threadIdx.x, l = lane_id();
-    const uint32_t K = T.K, S = T.S, Kpad = T.Kpad, Mp = T.Mp;
-    const uint32_t nw = (P.N_ad + 63) / 64;  // == Kpad / 64 (K == N_ad)
-    const uint32_t G = cref_group(nw);
-    for (uint32_t b = it.b0; b < it.b1; b++) {
-        uint32_t V[GPT][32];  // two uint16 values per register
-#pragma unroll
-        for (int e = 0; e < GPT; e++)
-#pragma unroll
-            for (int i = 0; i < 32; i++) V[e][i] = 0;
-        if (tid < 16) cnt[tid] = 0;
-        const uint32_t send = min(8 * b + 8, S + 1);
-        for (uint32_t sa = 8 * b; sa < send; sa += G) {
-            const uint32_t sb = min(sa + G, send);
-            cref_fill(P, T, sa, sb, nw, srow);
-            __syncthreads();
-            const uint32_t nrows = 2 * (sb - sa) - (sb == S + 1 ? 1u : 0u);
-            for (uint32_t rr = 0; rr < nrows; rr++) {
-                const uint32_t rb = 2 * (sa - 8 * b) + rr;
-                const uint64_t *row = srow + (uint64_t)rr * nw;
-                uint32_t ones = 0;
-#pragma unroll
-                for (int e = 0; e < GPT; e++) {
-                    const uint32_t q = tid + CREF_NT * e;
-                    const uint64_t wv = q < nw ? row[q] : 0ull;
-                    ones += (uint32_t)__popcll(wv);
-                    const uint32_t lo = (uint32_t)wv, hi = (uint32_t)(wv >> 32);
-#pragma unroll
-                    for (int i = 0; i < 16; i++) {
-                        // haplotypes 2i, 2i+1 (low word) and 32+2i, 33+2i (high word)
-                        V[e][i] |= (((lo >> (2 * i)) & 1u) << rb) | (((lo >> (2 * i + 1)) & 1u) << (16 + rb));
-                        V[e][16 + i] |= (((hi >> (2 * i)) & 1u) << rb) | (((hi >> (2 * i + 1)) & 1u) << (16 + rb));
-                    }
-                }
-                ones = wave_sum(ones);
-                if (l == 0) atomicAdd(&cnt[rb], ones);
-            }
-            __syncthreads();
-        }
-        uint16_t *__restrict__ dst = T.cref + (uint64_t)b * Kpad;
-#pragma unroll
-        for (int e = 0; e < GPT; e++) {
-            const uint32_t q = tid + CREF_NT * e;
-            if (q < nw) {
-                if (64 * q + 64 > K) {  // last group: haplotypes >= K are padding
-#pragma unroll
-                    for (int i = 0; i < 32; i++) {
-                        const uint32_t k0 = 64 * q + 2 * i;
-                        if (k0 >= K) V[e][i] |= 0xFFFFu;
-                        if (k0 + 1 >= K) V[e][i] |= 0xFFFF0000u;
-                    }
-                }
-                uint4 *d4 = (uint4 *)(dst + 64 * q);
-#pragma unroll
-                for (int i = 0; i < 8; i++)
-                    d4[i] = make_uint4(V[e][4 * i], V[e][4 * i + 1], V[e][4 * i + 2], V[e][4 * i + 3]);
-            }
-        }
-        if (tid < 16) {
-            const uint32_t r = 16 * b + tid;
-            if (r < Mp) T.count0[r] = (int32_t)K - (int32_t)cnt[tid];
-        }
-        __syncthreads();
-    }
+    for (uint32_t b = it.b0; b < it.b1; b++) cref_block_ident<GPT>(P, tg[it.t], b, srow, cnt);
 }
 
 // ---------------------------------------------------------------------------
 // a9: compressed PBWT.  One workgroup per (target, direction).
 //
-// Position i of the current order is held by wave i/(64*CPW), chunk
-// (i/64)%CPW, lane i%64, as a 32-bit payload (hap index << 16 | the 16 bits
-// of the current cref block for that haplotype).  Positions K..NPOS-1 are
-// padding whose row bits are all ones: under the stable partition (zeros
-// first) they never move, and they give the 1-padding of the last word.
-// A step:
-//   pass 1  ballot of the row bit per chunk -> the chunk's two data words go
-//           to output lane c, zero counts summed per wave, wave totals to LDS
-//   pass 2  running zeros-before / ones-before bases per chunk and the
-//           destination of each payload under the partition; payloads are
-//           scattered into LDS; output lanes derive their cumulative zero
-//           counts with a wave scan and store the row
+// The K positions of the current order are cut into chunks of 64; chunk ch is
+// held by wave ch % NW in slot ch / NW (so any K up to the variant's capacity
+// keeps all waves busy), one position per lane, as a 32-bit payload: hap
+// index << 16 | the 16 bits of the current cref block for that haplotype.
+// Positions K..Kpad-1 are padding whose row bits are all ones: under the
+// stable partition (zeros first) they never move, and they give the
+// 1-padding of the last word.  A step:
+//   pass 1  ballot of the row bit per slot: the slot's two data words go to
+//           output lane c; zero counts per chunk to LDS
+//   (barrier) each lane reads one row of chunk counts: zeros before every
+//           chunk by a wave scan, total zeros (count0)
+//   pass 2  destination of each payload under the partition from mbcnt of
+//           the ballot; payloads scattered into LDS (two contiguous runs per
+//           chunk); output lanes store their data / cumulative-zero pairs
+//   (barrier)
 //   pass 3  read back the payloads of my positions.
 // At each 16-row block boundary the next block's uint16 values are staged in
 // LDS and gathered by hap index.
@@ -677,35 +702,32 @@ __device__ __forceinline__ void static_for(F &&f) {
     }
 }
 
-template <int NT, int CPW>
-__global__ __launch_bounds__(NT) void k_pbwt(const TgtDev *__restrict__ tg,
-                                            const PbwtItem *__restrict__ items) {
+// NS = slots per wave for this target: chunks NS*NW > nch are padding
+// (all-ones bits, never moving), so every slot is processed unconditionally.
+template <int NT, int NS>
+__device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint32_t *smem) {
     constexpr int NW = NT / 64;
-    constexpr uint32_t NPOS = NW * CPW * 64;
-    static_assert(CPW <= 64, "one output lane per chunk");
-    extern __shared__ uint32_t smem[];
-    const PbwtItem it = items[blockIdx.x];
-    const TgtDev &T = tg[it.t];
+    constexpr uint32_t NPOS = NS * NW * 64;
+    static_assert(NS <= 64, "one output lane per slot");
     const uint32_t K = T.K, Mp = T.Mp, Kpad = T.Kpad, RS = T.RS;
     const uint32_t nch = Kpad / 64;
     const uint32_t npairs = (K + 31) / 32;
     const uint32_t KW = 2 * npairs;
     uint32_t *pay = smem;                                    // NPOS
     uint16_t *stage = (uint16_t *)(smem + NPOS);             // NPOS
-    uint32_t *tot = smem + NPOS + NPOS / 2;                  // 2*NW
+    uint32_t *zarr = smem + NPOS + NPOS / 2;                 // 2 x NS x NW chunk zero counts
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = lane_id();
-    const bool rev = it.dir != 0;
     const uint16_t *__restrict__ cref = T.cref;
     const bool dig = T.digest != nullptr;
 
-    // haplotypes beyond Kpad read all-ones rows from the stage forever
+    // haplotypes at positions >= Kpad read all-ones rows from the stage
     for (uint32_t q = Kpad + threadIdx.x; q < NPOS; q += NT) stage[q] = 0xFFFFu;
-    uint32_t p[CPW];
+    uint32_t p[NS];
     {
         const uint32_t b0 = (rev ? Mp - 1 : 0) >> 4;
 #pragma unroll
-        for (int c = 0; c < CPW; c++) {
-            const uint32_t pos = (wave * CPW + c) * 64 + ln;
+        for (int c = 0; c < NS; c++) {
+            const uint32_t pos = ((uint32_t)c * NW + wave) * 64 + ln;
             // the cref kernel writes all-ones for K <= k < Kpad
             const uint32_t v = pos < Kpad ? (uint32_t)cref[(uint64_t)b0 * Kpad + pos] : 0xFFFFu;
             p[c] = (pos << 16) | v;
@@ -714,9 +736,9 @@ __global__ __launch_bounds__(NT) void k_pbwt(const TgtDev *__restrict__ tg,
     uint32_t *__restrict__ outbase = T.out + (rev ? (uint64_t)Mp * RS : 0ull);
     const uint64_t gbase = rev ? (uint64_t)Mp * KW : 0ull;
     uint64_t dsum = 0;
-    const uint32_t nslots = RS / 4;  // 4-word slots of a stored row
-    const uint32_t och = wave * CPW + ln;  // chunk stored by this lane (ln < CPW)
-    const bool ostore = ln < (uint32_t)CPW && och < nch;
+    const uint32_t nslots = RS / 4;        // 4-word slots of a stored row
+    const uint32_t och = ln * NW + wave;   // chunk stored by this lane
+    const bool ostore = ln < (uint32_t)NS && och < nch;
     const uint32_t ln4 = 4 * ln;
 
     for (uint32_t step = 0; step < Mp; ++step) {
@@ -728,62 +750,61 @@ __global__ __launch_bounds__(NT) void k_pbwt(const TgtDev *__restrict__ tg,
             for (uint32_t q = threadIdx.x; q < Kpad / 2; q += NT) st32[q] = src[q];
             __syncthreads();
 #pragma unroll
-            for (int c = 0; c < CPW; c++) {
+            for (int c = 0; c < NS; c++) {
                 const uint32_t id = p[c] >> 16;
                 p[c] = (id << 16) | (uint32_t)stage[id];
             }
         }
-        // pass 1: chunk c's ballot words go to lane c
+        // pass 1: slot c's ballot words go to lane c
         uint32_t olo = 0, ohi = 0;
-        static_for<0, CPW>([&](auto ci) {
+        static_for<0, NS>([&](auto ci) {
             constexpr int c = decltype(ci)::value;
             const uint64_t m = __ballot((p[c] >> j) & 1u);
             olo = writelane<c>(olo, (uint32_t)m);
             ohi = writelane<c>(ohi, (uint32_t)(m >> 32));
         });
-        // zeros per chunk, and their prefix over this wave's chunks
-        const uint32_t zc = ln < (uint32_t)CPW ? 64u - (uint32_t)__popc(olo) - (uint32_t)__popc(ohi) : 0u;
-        const uint32_t zinc = wave_scan_incl(zc);
-        const uint32_t par = step & 1u;
-        if (ln == 0) tot[par * NW + wave] = __builtin_amdgcn_readlane(zinc, 63);
+        const uint32_t zc = ln < (uint32_t)NS ? 64u - (uint32_t)__popc(olo) - (uint32_t)__popc(ohi) : 0u;
+        uint32_t *zp = zarr + (step & 1u) * (NS * NW);
+        if (ln < (uint32_t)NS) zp[ln * NW + wave] = zc;
         __syncthreads();
-        uint32_t base = 0, cnt0 = 0;
+        // lane c: zeros in slot row c (all waves) and in the waves before mine
+        uint32_t rowtot = 0, inrow = 0;
+        if (ln < (uint32_t)NS) {
 #pragma unroll
-        for (int w2 = 0; w2 < NW; w2++) {
-            const uint32_t v = tot[par * NW + w2];
-            cnt0 += v;
-            base += (w2 < (int)wave) ? v : 0u;
+            for (int w2 = 0; w2 < NW; w2++) {
+                const uint32_t v = zp[ln * NW + w2];
+                rowtot += v;
+                inrow += (w2 < (int)wave) ? v : 0u;
+            }
         }
-        base = __builtin_amdgcn_readfirstlane(base);
-        cnt0 = __builtin_amdgcn_readfirstlane(cnt0);
+        const uint32_t rowinc = wave_scan_incl(rowtot);
+        const uint32_t cnt0 = __builtin_amdgcn_readlane(rowinc, 63);
+        const uint32_t zbv = rowinc - rowtot + inrow;  // zeros before chunk och
         // pass 2: stable partition, zeros first; byte addresses into pay[]
-        uint32_t zrun4 = 4 * base;                          // zeros before the chunk, x4
-        uint32_t ob4 = 4 * (cnt0 + wave * CPW * 64 - base);  // ones before the chunk, x4
-#pragma unroll
-        for (int c = 0; c < CPW; c++) {
+        static_for<0, NS>([&](auto ci) {
+            constexpr int c = decltype(ci)::value;
+            const uint32_t zb = __builtin_amdgcn_readlane(zbv, c);
+            const uint32_t ch = (uint32_t)c * NW + wave;
+            const uint32_t zb4 = 4 * zb, ob4 = 4 * (cnt0 + ch * 64 - zb);
             uint32_t b = (p[c] >> j) & 1u;
-            // recompute the ballot here rather than keep CPW masks live across
+            // recompute the ballot here rather than keep NS masks live across
             // the barrier (they would spill)
             asm volatile("" : "+v"(b));
             const uint64_t m = __ballot(b != 0);
             const uint32_t t1 = mbcnt(m);
-            const uint32_t zd4 = zrun4 + ln4 - 4 * t1;
+            const uint32_t zd4 = zb4 + ln4 - 4 * t1;
             const uint32_t od4 = ob4 + 4 * t1;
             // unpredictable: keep a v_cndmask (a plain select may become a branch)
             const uint32_t dest4 = __builtin_unpredictable(b != 0) ? od4 : zd4;
             *(uint32_t *)((char *)pay + dest4) = p[c];
-            const uint32_t ones = (uint32_t)__popcll(m);
-            zrun4 += 4 * (64u - ones);
-            ob4 += 4 * ones;
-        }
-        const uint32_t zb = base + zinc - zc;  // zeros before chunk (lanes < CPW)
+        });
         __syncthreads();
 #pragma unroll
-        for (int c = 0; c < CPW; c++) p[c] = pay[(wave * CPW + c) * 64 + ln];
+        for (int c = 0; c < NS; c++) p[c] = pay[((uint32_t)c * NW + wave) * 64 + ln];
         // the row: chunk ch holds pairs 2ch and 2ch+1
         uint32_t *__restrict__ row = outbase + (uint64_t)step * RS;
         if (ostore) {
-            const uint32_t o1 = zb + 32u - (uint32_t)__popc(olo);
+            const uint32_t o1 = zbv + 32u - (uint32_t)__popc(olo);
             const uint32_t o3 = o1 + 32u - (uint32_t)__popc(ohi);
             const bool second = 2 * och + 1 < npairs;
             uint2 *dst = (uint2 *)(row + 4 * och);
@@ -797,7 +818,7 @@ __global__ __launch_bounds__(NT) void k_pbwt(const TgtDev *__restrict__ tg,
             }
         }
         // zero slots after the logical row
-        const uint32_t tail = nch + wave * 64 + ln;
+        const uint32_t tail = nch + threadIdx.x;
         if (tail < nslots) ((uint4 *)row)[tail] = make_uint4(0u, 0u, 0u, 0u);
     }
     if (dig) {
@@ -806,10 +827,222 @@ __global__ __launch_bounds__(NT) void k_pbwt(const TgtDev *__restrict__ tg,
     }
 }
 
+// slot counts instantiated; a target uses the smallest one >= ceil(chunks / waves)
+constexpr int kSlotCounts[] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 14, 16, 20, 24, 28, 32};
+constexpr int kNumSlotCounts = sizeof(kSlotCounts) / sizeof(kSlotCounts[0]);
+
+__host__ __device__ inline int pbwt_slots_for(uint32_t Kpad, int nt) {
+    const uint32_t need = (Kpad / 64 + nt / 64 - 1) / (nt / 64);
+    for (int i = 0; i < kNumSlotCounts; i++)
+        if ((uint32_t)kSlotCounts[i] >= need) return kSlotCounts[i];
+    return -1;
+}
+
+template <int NT, int CPW>
+__device__ __forceinline__ void pbwt_target(const TgtDev &T, const bool rev, uint32_t *smem) {
+    const int ns = pbwt_slots_for(T.Kpad, NT);
+    static_for<0, kNumSlotCounts>([&](auto ii) {
+        constexpr int NSv = kSlotCounts[decltype(ii)::value];
+        if constexpr (NSv <= CPW) {
+            if (ns == NSv) pbwt_steps<NT, NSv>(T, rev, smem);
+        }
+    });
+}
+
+template <int NT, int CPW>
+__global__ __launch_bounds__(NT) void k_pbwt(const TgtDev *__restrict__ tg,
+                                            const PbwtItem *__restrict__ items) {
+    extern __shared__ uint32_t smem[];
+    const PbwtItem it = items[blockIdx.x];
+    pbwt_target<NT, CPW>(tg[it.t], it.dir != 0, smem);
+}
+
+// ---------------------------------------------------------------------------
+// Device-driven batch (stream contexts): k_plan sizes every target's outputs
+// from its split count, lays them out in the context's pools (targets that do
+// not fit are deferred), and queues cref blocks and PBWT items (largest
+// first) for the persistent kernels below.  One workgroup, n <= 1024.
+// ---------------------------------------------------------------------------
+constexpr int PLAN_NT = 1024;
+
+__device__ __forceinline__ uint64_t wave_scan_incl64(uint64_t v) {
+    const uint32_t l = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t u = __shfl_up(v, o);
+        if (l >= (uint32_t)o) v += u;
+    }
+    return v;
+}
+
+// block-wide exclusive scan of 64-bit values (PLAN_NT threads); total returned
+__device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t *s_w, uint64_t &total) {
+    const uint32_t wave = threadIdx.x >> 6, l = lane_id();
+    const uint64_t inc = wave_scan_incl64(v);
+    if (l == 63) s_w[wave] = inc;
+    __syncthreads();
+    uint64_t before = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < PLAN_NT / 64; w++) {
+        const uint64_t x = s_w[w];
+        tot += x;
+        if (w < (int)wave) before += x;
+    }
+    __syncthreads();
+    total = tot;
+    return before + inc - v;
+}
+
+__global__ __launch_bounds__(PLAN_NT) void k_plan(TgtDev *__restrict__ tg, BatchDev *__restrict__ B,
+                                                 PlanArgs A) {
+    __shared__ uint64_t s_w[PLAN_NT / 64];
+    __shared__ uint64_t keys[2 * PLAN_NT];
+    const uint32_t t = threadIdx.x;
+    const uint32_t n = B->n;
+    const bool valid = t < n;
+    uint32_t Mp = 0, Kpad = 0, RS = 0, nblk = 0, S = 0, K = 0;
+    uint64_t cb = 0, c0b = 0, ob = 0;
+    if (valid) {
+        const TgtDev &T = tg[t];
+        S = T.stats[0];
+        K = T.K;
+        Mp = 2 * S + 1;
+        Kpad = (K + 63) / 64 * 64;
+        RS = A.compact ? 2 * ((K + 31) / 32) : 4 * (((K + 511) / 512) * 8);
+        nblk = (Mp + 15) / 16;
+        cb = region_cref_bytes(Mp, Kpad);
+        c0b = region_c0_bytes(Mp);
+        ob = region_out_bytes(Mp, RS);
+    }
+    uint64_t tot;
+    const uint64_t off = block_excl_scan64(cb + c0b + ob, s_w, tot);
+    // prefix sums grow with t, so the targets that fit are a prefix of the batch
+    const bool fit = valid && off + cb + c0b + ob <= A.pool_cap;
+    if (valid) {
+        TgtDev &T = tg[t];
+        T.S = S;
+        T.stats[2] = fit ? 0u : 1u;  // 1: deferred, outputs not built
+        if (fit) {
+            T.Mp = Mp;
+            T.Kpad = Kpad;
+            T.RS = RS;
+            T.cref = (uint16_t *)(A.pool + off);
+            T.count0 = (int32_t *)(A.pool + off + cb);
+            T.out = (uint32_t *)(A.pool + off + cb + c0b);
+            T.digest = A.digest ? A.dig_base + 2 * t : nullptr;
+            if (A.digest) {
+                A.dig_base[2 * t] = 0;
+                A.dig_base[2 * t + 1] = 0;
+            }
+        } else {
+            T.Mp = 0;
+            T.digest = nullptr;
+        }
+    }
+    uint64_t nb_tot;
+    const uint64_t bp = block_excl_scan64(fit ? nblk : 0u, s_w, nb_tot);
+    if (valid) A.blockpref[t] = (uint32_t)bp;
+    uint64_t nfit;
+    (void)block_excl_scan64(fit ? 1u : 0u, s_w, nfit);
+    if (t == 0) {
+        A.blockpref[n] = (uint32_t)nb_tot;
+        B->ncref = (uint32_t)nb_tot;
+        B->npbwt = 2 * (uint32_t)nfit;
+        B->nfit = (uint32_t)nfit;
+        B->cref_ctr = 0;
+        B->pbwt_ctr = 0;
+    }
+    // PBWT items 2t (forward) and 2t+1 (backward), most work first:
+    // key = (~(Mp * Kpad) & 40 bits) << 24 | item; unused slots sort last
+    const uint64_t wk = fit ? ((~((uint64_t)Mp * Kpad)) & 0xFFFFFFFFFFull) << 24 : ~0ull;
+    keys[2 * t] = fit ? (wk | (2 * t)) : ~0ull;
+    keys[2 * t + 1] = fit ? (wk | (2 * t + 1)) : ~0ull;
+    __syncthreads();
+    // bitonic sort of 2*PLAN_NT keys, ascending
+    for (uint32_t k = 2; k <= 2 * PLAN_NT; k <<= 1) {
+        for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+            for (uint32_t i = t; i < 2 * PLAN_NT; i += PLAN_NT) {
+                const uint32_t ixj = i ^ jj;
+                if (ixj > i) {
+                    const uint64_t a = keys[i], b = keys[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((a > b) == up) {
+                        keys[i] = b;
+                        keys[ixj] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t i = t; i < 2 * (uint32_t)nfit; i += PLAN_NT) {
+        const uint32_t item = (uint32_t)(keys[i] & 0xFFFFFFu);
+        A.pitems[i] = PbwtItem{item >> 1, item & 1u};
+    }
+}
+
+// cref blocks from the queue; both target kinds (identity / subset)
+template <int EPT, int GPT>
+__global__ __launch_bounds__(CREF_NT) void k_cref_persist(PanelDev P, const TgtDev *__restrict__ tg,
+                                                         BatchDev *__restrict__ B,
+                                                         const uint32_t *__restrict__ blockpref,
+                                                         int ident_ok) {
+    extern __shared__ uint64_t srow[];
+    __shared__ uint32_t cnt[16];
+    __shared__ uint32_t s_i, s_t;
+    const uint32_t ncref = B->ncref, n = B->n;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const uint32_t i = atomicAdd(&B->cref_ctr, 1u);
+            uint32_t t = 0;
+            if (i < ncref) {  // last target whose first block is <= i
+                uint32_t lo = 0, hi = n;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (blockpref[mid] <= i) lo = mid; else hi = mid;
+                }
+                t = lo;
+            }
+            s_i = i;
+            s_t = t;
+        }
+        __syncthreads();
+        const uint32_t i = s_i, t = s_t;
+        __syncthreads();
+        if (i >= ncref) break;
+        const TgtDev &T = tg[t];
+        const uint32_t b = i - blockpref[t];
+        if (ident_ok && (T.flags & TF_IDENTITY)) cref_block_ident<GPT>(P, T, b, srow, cnt);
+        else cref_block_gather<EPT>(P, T, b, srow, cnt);
+    }
+}
+
+// PBWT items from the queue
+template <int NT, int CPW>
+__global__ __launch_bounds__(NT) void k_pbwt_persist(const TgtDev *__restrict__ tg,
+                                                    BatchDev *__restrict__ B,
+                                                    const PbwtItem *__restrict__ items) {
+    extern __shared__ uint32_t smem[];
+    __shared__ uint32_t s_i;
+    const uint32_t np = B->npbwt;
+    for (;;) {
+        if (threadIdx.x == 0) s_i = atomicAdd(&B->pbwt_ctr, 1u);
+        __syncthreads();
+        const uint32_t i = s_i;
+        __syncthreads();
+        if (i >= np) break;
+        const PbwtItem it = items[i];
+        pbwt_target<NT, CPW>(tg[it.t], it.dir != 0, smem);
+        __syncthreads();
+    }
+}
+
 // count0 digest (grid n, block 256)
-__global__ __launch_bounds__(256) void k_digest_count0(const TgtDev *__restrict__ tg) {
+__global__ __launch_bounds__(256) void k_digest_count0(const TgtDev *__restrict__ tg,
+                                                      const uint32_t *__restrict__ d_n) {
+    if (d_n && blockIdx.x >= *d_n) return;
     const TgtDev &T = tg[blockIdx.x];
-    if (!T.digest) return;
+    if (!T.digest || T.Mp == 0) return;
     uint64_t s = 0;
     for (uint32_t r = threadIdx.x; r < T.Mp; r += 256) s += dig_count0(r, T.count0[r]);
     s = wave_sum64(s);
@@ -835,6 +1068,19 @@ __global__ __launch_bounds__(256) void k_cref_rows(const TgtDev *__restrict__ tg
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+// The dynamic-LDS limit of a kernel is process-wide: contexts of different
+// sizes (and engines with different panels) only ever raise it.
+static hipError_t raise_lds_limit(const void *fn, size_t lds) {
+    static std::mutex mu;
+    static std::map<const void *, size_t> limit;
+    std::lock_guard<std::mutex> lk(mu);
+    size_t &cur = limit[fn];
+    if (lds <= cur) return hipSuccess;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e == hipSuccess) cur = lds;
+    return e;
+}
+
 hipError_t launch_site_to_hap(const PanelDev &P, uint64_t *hap, hipStream_t s) {
     dim3 grid((unsigned)((P.hap_stride + 3) / 4), (P.N_ad + 63) / 64);
     hipLaunchKernelGGL(k_site_to_hap, grid, dim3(256), 0, s, P, hap);
@@ -847,24 +1093,26 @@ hipError_t launch_panel_mono(const PanelDev &P, uint64_t *pand, uint64_t *por, h
 hipError_t launch_distance(const PanelDev &P, const TgtDev *tg, const uint32_t *tlist,
                            uint32_t nsel, hipStream_t s) {
     if (!nsel) return hipSuccess;
-    hipLaunchKernelGGL(k_distance, dim3((P.N_ad + 15) / 16, nsel), dim3(256), 0, s, P, tg, tlist);
+    hipLaunchKernelGGL(k_distance, dim3((P.N_ad + 15) / 16, nsel), dim3(256), 0, s, P, tg, tlist,
+                       (const uint32_t *)nullptr);
     return hipGetLastError();
 }
 hipError_t launch_select(const PanelDev &P, const TgtDev *tg, uint32_t n, hipStream_t s) {
     if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_select, dim3(n), dim3(SEL_NT), 0, s, P, tg);
+    hipLaunchKernelGGL(k_select, dim3(n), dim3(SEL_NT), 0, s, P, tg, (const uint32_t *)nullptr);
     return hipGetLastError();
 }
 hipError_t launch_monos(const PanelDev &P, const TgtDev *tg, const uint32_t *tlist, uint32_t n,
                         uint32_t max_ww, hipStream_t s) {
     if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_monos, dim3((max_ww + 63) / 64, n), dim3(1024), 0, s, P, tg, tlist);
+    hipLaunchKernelGGL(k_monos, dim3((max_ww + 63) / 64, n), dim3(1024), 0, s, P, tg, tlist,
+                       (const uint32_t *)nullptr);
     return hipGetLastError();
 }
 hipError_t launch_splits(const PanelDev &P, const TgtDev *tg, uint32_t n, hipStream_t s) {
     if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_splits_clear, dim3(n), dim3(64), 0, s, tg);
-    hipLaunchKernelGGL(k_splits, dim3(n), dim3(64), 0, s, P, tg);
+    hipLaunchKernelGGL(k_splits_clear, dim3(n), dim3(64), 0, s, tg, (const uint32_t *)nullptr);
+    hipLaunchKernelGGL(k_splits, dim3(n), dim3(64), 0, s, P, tg, (const uint32_t *)nullptr);
     return hipGetLastError();
 }
 hipError_t launch_bits_to_list(const TgtDev *tg, uint32_t n, uint32_t nbits_bh, int which,
@@ -876,6 +1124,8 @@ hipError_t launch_bits_to_list(const TgtDev *tg, uint32_t n, uint32_t nbits_bh, 
 
 #define CREF_CASE(E)                                                                      \
     if (ept <= E) {                                                                       \
+        hipError_t e_ = raise_lds_limit((const void *)k_cref<E>, lds);                    \
+        if (e_ != hipSuccess) return e_;                                                  \
         hipLaunchKernelGGL(k_cref<E>, dim3(nitems), dim3(CREF_NT), lds, s, P, tg, items); \
         return hipGetLastError();                                                         \
     }
@@ -886,12 +1136,14 @@ hipError_t launch_cref(const PanelDev &P, const TgtDev *tg, const CrefItem *item
     const size_t lds = (size_t)2 * cref_group(nw) * nw * sizeof(uint64_t);
     if (ident) {
         const uint32_t gpt = (nw + CREF_NT - 1) / CREF_NT;
-        if (gpt <= 1) {
-            hipLaunchKernelGGL(k_cref_ident<1>, dim3(nitems), dim3(CREF_NT), lds, s, P, tg, items);
-            return hipGetLastError();
-        }
         if (gpt <= 2) {
-            hipLaunchKernelGGL(k_cref_ident<2>, dim3(nitems), dim3(CREF_NT), lds, s, P, tg, items);
+            const void *fn = gpt <= 1 ? (const void *)k_cref_ident<1> : (const void *)k_cref_ident<2>;
+            hipError_t e = raise_lds_limit(fn, lds);
+            if (e != hipSuccess) return e;
+            if (gpt <= 1)
+                hipLaunchKernelGGL(k_cref_ident<1>, dim3(nitems), dim3(CREF_NT), lds, s, P, tg, items);
+            else
+                hipLaunchKernelGGL(k_cref_ident<2>, dim3(nitems), dim3(CREF_NT), lds, s, P, tg, items);
             return hipGetLastError();
         }
     }
@@ -911,7 +1163,7 @@ uint32_t cref_max_k() { return 96 * CREF_NT; }
 
 hipError_t launch_digest_count0(const TgtDev *tg, uint32_t n, hipStream_t s) {
     if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_digest_count0, dim3(n), dim3(256), 0, s, tg);
+    hipLaunchKernelGGL(k_digest_count0, dim3(n), dim3(256), 0, s, tg, (const uint32_t *)nullptr);
     return hipGetLastError();
 }
 hipError_t launch_cref_rows(const TgtDev *tg, uint32_t t, uint32_t Mp, uint64_t *rows, uint32_t rw,
@@ -920,18 +1172,18 @@ hipError_t launch_cref_rows(const TgtDev *tg, uint32_t t, uint32_t Mp, uint64_t 
     return hipGetLastError();
 }
 
-// PBWT variants: (threads, chunks per wave)
+// PBWT variants: (threads, chunks per wave), one-item-per-workgroup and queue kernels
 struct PbwtVariant {
     int nt, cpw;
     void (*fn)(const TgtDev *, const PbwtItem *);
+    void (*fn_persist)(const TgtDev *, BatchDev *, const PbwtItem *);
 };
+#define PBWT_VARIANT(NT, CPW) {NT, CPW, k_pbwt<NT, CPW>, k_pbwt_persist<NT, CPW>}
 static const PbwtVariant kPbwtVariants[] = {
-    {256, 4, k_pbwt<256, 4>},    {256, 8, k_pbwt<256, 8>},    {256, 16, k_pbwt<256, 16>},
-    {256, 24, k_pbwt<256, 24>},  {256, 32, k_pbwt<256, 32>},  {256, 40, k_pbwt<256, 40>},
-    {512, 8, k_pbwt<512, 8>},    {512, 16, k_pbwt<512, 16>},  {512, 20, k_pbwt<512, 20>},
-    {512, 24, k_pbwt<512, 24>},  {512, 32, k_pbwt<512, 32>},  {1024, 8, k_pbwt<1024, 8>},
-    {1024, 12, k_pbwt<1024, 12>}, {1024, 16, k_pbwt<1024, 16>}, {1024, 24, k_pbwt<1024, 24>},
+    PBWT_VARIANT(256, 8),  PBWT_VARIANT(256, 16),  PBWT_VARIANT(512, 20),
+    PBWT_VARIANT(512, 32), PBWT_VARIANT(1024, 24),
 };
+#undef PBWT_VARIANT
 constexpr int kNumPbwtVariants = sizeof(kPbwtVariants) / sizeof(kPbwtVariants[0]);
 
 int pbwt_variant_count() { return kNumPbwtVariants; }
@@ -941,19 +1193,112 @@ uint32_t pbwt_variant_maxk(int v) {
     return (uint32_t)(kPbwtVariants[v].nt / 64) * kPbwtVariants[v].cpw * 64;
 }
 size_t pbwt_lds_bytes(int v, uint32_t Kpad) {
-    (void)Kpad;  // the kernel covers all NT/64*CPW*64 positions
-    const size_t npos = pbwt_variant_maxk(v);
-    return npos * 4 + npos * 2 + 2 * (kPbwtVariants[v].nt / 64) * 4;
+    const int nt = kPbwtVariants[v].nt;
+    const int ns = pbwt_slots_for(Kpad, nt);
+    if (ns < 0 || ns > kPbwtVariants[v].cpw) return ~(size_t)0;
+    const size_t npos = (size_t)ns * nt;  // NS x NW x 64
+    return npos * 4 + npos * 2 + 2ull * ns * (nt / 64) * 4;
 }
 hipError_t pbwt_prepare(int v, size_t lds) {
-    return hipFuncSetAttribute((const void *)kPbwtVariants[v].fn,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    return raise_lds_limit((const void *)kPbwtVariants[v].fn, lds);
 }
 hipError_t launch_pbwt(int v, const TgtDev *tg, const PbwtItem *items, uint32_t nitems, size_t lds,
                        hipStream_t s) {
     if (!nitems) return hipSuccess;
     hipLaunchKernelGGL(kPbwtVariants[v].fn, dim3(nitems), dim3(kPbwtVariants[v].nt), lds, s, tg,
                        items);
+    return hipGetLastError();
+}
+
+// ---- device-driven batch (stream contexts)
+
+// Prep phase over a batch of *d_n <= nmax targets, one kernel class per call
+// (0 distances, 1 top-K, 2 monos, 3 split sites); grids cover nmax targets.
+hipError_t launch_prep_dev(int cls, const PanelDev &P, const TgtDev *tg, const uint32_t *d_n,
+                           uint32_t nmax, uint32_t max_ww, hipStream_t s) {
+    switch (cls) {
+    case 0:
+        hipLaunchKernelGGL(k_distance, dim3((P.N_ad + 15) / 16, nmax), dim3(256), 0, s, P, tg,
+                           (const uint32_t *)nullptr, d_n);
+        break;
+    case 1:
+        hipLaunchKernelGGL(k_select, dim3(nmax), dim3(SEL_NT), 0, s, P, tg, d_n);
+        break;
+    case 2:
+        hipLaunchKernelGGL(k_monos, dim3((max_ww + 63) / 64, nmax), dim3(1024), 0, s, P, tg,
+                           (const uint32_t *)nullptr, d_n);
+        break;
+    default:
+        hipLaunchKernelGGL(k_splits_clear, dim3(nmax), dim3(64), 0, s, tg, d_n);
+        hipLaunchKernelGGL(k_splits, dim3(nmax), dim3(64), 0, s, P, tg, d_n);
+        break;
+    }
+    return hipGetLastError();
+}
+hipError_t launch_plan(TgtDev *tg, BatchDev *B, const PlanArgs &A, hipStream_t s) {
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(PLAN_NT), 0, s, tg, B, A);
+    return hipGetLastError();
+}
+hipError_t launch_digest_count0_dev(const TgtDev *tg, const uint32_t *d_n, uint32_t nmax,
+                                    hipStream_t s) {
+    hipLaunchKernelGGL(k_digest_count0, dim3(nmax), dim3(256), 0, s, tg, d_n);
+    return hipGetLastError();
+}
+
+using CrefPersistFn = void (*)(PanelDev, const TgtDev *, BatchDev *, const uint32_t *, int);
+#define CREF_P_CASE(E)                                                                  \
+    if (ept <= E) return gpt <= 1 ? k_cref_persist<E, 1> : k_cref_persist<E, 2>;
+static CrefPersistFn cref_persist_fn(uint32_t maxK, uint32_t gpt) {
+    const uint32_t ept = (maxK + CREF_NT - 1) / CREF_NT;
+    CREF_P_CASE(4)
+    CREF_P_CASE(8)
+    CREF_P_CASE(16)
+    CREF_P_CASE(32)
+    CREF_P_CASE(48)
+    CREF_P_CASE(64)
+    CREF_P_CASE(96)
+    return nullptr;
+}
+#undef CREF_P_CASE
+
+size_t cref_lds_bytes(const PanelDev &P) {
+    const uint32_t nw = (P.N_ad + 63) / 64;
+    return (size_t)2 * cref_group(nw) * nw * sizeof(uint64_t);
+}
+// identity targets take the transpose path when every 64-haplotype group
+// fits two per thread
+bool cref_ident_ok(const PanelDev &P) { return (P.N_ad + 63) / 64 <= 2u * CREF_NT; }
+
+// sets the dynamic LDS limit; *occ = resident workgroups per CU
+hipError_t cref_persist_prepare(const PanelDev &P, uint32_t maxK, int *occ) {
+    const uint32_t nw = (P.N_ad + 63) / 64;
+    CrefPersistFn fn = cref_persist_fn(maxK, (nw + CREF_NT - 1) / CREF_NT);
+    if (!fn) return hipErrorInvalidValue;
+    const size_t lds = cref_lds_bytes(P);
+    hipError_t e = raise_lds_limit((const void *)fn, lds);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, (const void *)fn, CREF_NT, lds);
+}
+hipError_t launch_cref_persist(const PanelDev &P, const TgtDev *tg, BatchDev *B,
+                               const uint32_t *blockpref, uint32_t maxK, uint32_t grid,
+                               hipStream_t s) {
+    const uint32_t nw = (P.N_ad + 63) / 64;
+    CrefPersistFn fn = cref_persist_fn(maxK, (nw + CREF_NT - 1) / CREF_NT);
+    if (!fn) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(CREF_NT), cref_lds_bytes(P), s, P, tg, B, blockpref,
+                       cref_ident_ok(P) ? 1 : 0);
+    return hipGetLastError();
+}
+hipError_t pbwt_persist_prepare(int v, size_t lds, int *occ) {
+    const void *fn = (const void *)kPbwtVariants[v].fn_persist;
+    hipError_t e = raise_lds_limit(fn, lds);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, fn, kPbwtVariants[v].nt, lds);
+}
+hipError_t launch_pbwt_persist(int v, const TgtDev *tg, BatchDev *B, const PbwtItem *items,
+                               uint32_t grid, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL(kPbwtVariants[v].fn_persist, dim3(grid), dim3(kPbwtVariants[v].nt), lds, s,
+                       tg, B, items);
     return hipGetLastError();
 }
 

@@ -17,6 +17,9 @@ LIB_PATH = os.path.join(HERE, "libeimp.so")
 
 EIMP_BATCH_DIGEST = 1
 EIMP_BATCH_COMPACT = 4
+EIMP_STREAM_NO_GRAPH = 8
+EIMP_STREAM_TIMED = 16
+EIMP_BUSY = 1
 KCLASSES = ["distance", "select", "monos", "splits", "cref", "pbwt", "copy"]
 
 # every entry point declared in include/eimp.h
@@ -26,7 +29,9 @@ EXPORTS = [
     "eimp_engine_free", "eimp_engine_device_index", "eimp_engine_set_panel_extras",
     "eimp_engine_run_batch", "eimp_engine_copy_target", "eimp_engine_set_timing",
     "eimp_engine_get_timing", "eimp_engine_reset_timing", "eimp_engine_set_pbwt_variant",
-    "eimp_pbwt_variant_count", "eimp_pbwt_variant_info",
+    "eimp_pbwt_variant_count", "eimp_pbwt_variant_info", "eimp_stream_create",
+    "eimp_stream_submit", "eimp_stream_query", "eimp_stream_wait", "eimp_stream_copy_target",
+    "eimp_stream_destroy", "eimp_engine_mem_info", "eimp_stream_kernel_ms",
 ]
 
 _P = C.c_void_p
@@ -71,6 +76,14 @@ def lib():
         L.eimp_pbwt_variant_count.argtypes = []
         L.eimp_pbwt_variant_info.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                              C.POINTER(C.c_int)]
+        L.eimp_stream_create.argtypes = [_P, _SZ, _SZ, _SZ, _SZ, C.c_uint32, C.POINTER(_P)]
+        L.eimp_stream_submit.argtypes = [_P, _SZ, _P, C.c_float]
+        L.eimp_stream_query.argtypes = [_P]
+        L.eimp_stream_wait.argtypes = [_P, _P, C.POINTER(_SZ), _P]
+        L.eimp_stream_copy_target.argtypes = [_P, _SZ, _P, _P, _P, _P, _P, _P, _P, _P]
+        L.eimp_stream_destroy.argtypes = [_P]
+        L.eimp_engine_mem_info.argtypes = [_P, C.POINTER(_SZ), C.POINTER(_SZ)]
+        L.eimp_stream_kernel_ms.argtypes = [_P, _P]
         _LIB = L
     return _LIB
 
@@ -212,27 +225,19 @@ class Engine:
 
     def copy_target(self, t, cref=False) -> dict:
         S, Mp, _, K = (int(x) for x in self._last[t, :4])
-        W = self._last_W[t]
-        rs = 2 * ((K + 31) // 32) if self._last_compact else 4 * unit_words(K)
-        out = dict(
-            besthaps=np.zeros(K, np.uint32),
-            bhflags=np.zeros(unit_words(self.Nad), np.uint64),
-            splitsites=np.zeros(max(S, 1), np.uint32),
-            splits=np.zeros(unit_words(W), np.uint64),
-            cMdiffs=np.zeros(S + 1 if S else 1, np.float32),
-            count0=np.zeros(Mp, np.int32),
-            pbwt=np.zeros(2 * Mp * rs, np.uint32),
-        )
-        cr = np.zeros(Mp * unit_words(K), np.uint64) if cref else None
-        _check(lib().eimp_engine_copy_target(
-            self._h, t, _p(out["besthaps"]), _p(out["bhflags"]), _p(out["splitsites"]),
-            _p(out["splits"]), _p(out["cMdiffs"]), _p(out["count0"]), _p(out["pbwt"]), _p(cr)))
-        out["splitsites"] = out["splitsites"][:S]
-        out["cMdiffs"] = out["cMdiffs"][:S + 1] if S else out["cMdiffs"][:0]
-        out["pbwt"] = out["pbwt"].reshape(2 * Mp, rs)
-        if cref:
-            out["cref"] = cr.reshape(Mp, unit_words(K))
-        return out
+        return _copy_out(lib().eimp_engine_copy_target, self._h, t, S, Mp, K, self._last_W[t],
+                         self._last_compact, self.Nad, cref)
+
+    def mem_info(self):
+        """(free, total) device bytes."""
+        f, t = _SZ(), _SZ()
+        _check(lib().eimp_engine_mem_info(self._h, C.byref(f), C.byref(t)))
+        return f.value, t.value
+
+    def stream(self, max_targets, max_window, max_K, pool_bytes, digest=True, compact=False,
+               graph=True, timed=False) -> "Stream":
+        return Stream(self, max_targets, max_window, max_K, pool_bytes, digest, compact, graph,
+                      timed)
 
     # -- measurement
     def set_timing(self, on=True):
@@ -249,3 +254,108 @@ class Engine:
 
     def set_pbwt_variant(self, v: int):
         _check(lib().eimp_engine_set_pbwt_variant(self._h, v))
+
+
+def _copy_out(fn, h, t, S, Mp, K, W, compact, Nad, cref) -> dict:
+    rs = 2 * ((K + 31) // 32) if compact else 4 * unit_words(K)
+    out = dict(
+        besthaps=np.zeros(K, np.uint32),
+        bhflags=np.zeros(unit_words(Nad), np.uint64),
+        splitsites=np.zeros(max(S, 1), np.uint32),
+        splits=np.zeros(unit_words(W), np.uint64),
+        cMdiffs=np.zeros(S + 1 if S else 1, np.float32),
+        count0=np.zeros(Mp, np.int32),
+        pbwt=np.zeros(2 * Mp * rs, np.uint32),
+    )
+    cr = np.zeros(Mp * unit_words(K), np.uint64) if cref else None
+    _check(fn(h, t, _p(out["besthaps"]), _p(out["bhflags"]), _p(out["splitsites"]),
+              _p(out["splits"]), _p(out["cMdiffs"]), _p(out["count0"]), _p(out["pbwt"]), _p(cr)))
+    out["splitsites"] = out["splitsites"][:S]
+    out["cMdiffs"] = out["cMdiffs"][:S + 1] if S else out["cMdiffs"][:0]
+    out["pbwt"] = out["pbwt"].reshape(2 * Mp, rs)
+    if cref:
+        out["cref"] = cr.reshape(Mp, unit_words(K))
+    return out
+
+
+class Stream:
+    """Asynchronous device-driven batches on their own HIP stream (eimp_stream_*).
+
+    submit() returns at once; wait() collects the per-target summaries.  The
+    launch sequence is a hipGraph captured at creation (graph=False launches
+    the same kernels one by one; timed=True also records events between the
+    kernel classes, read with kernel_ms())."""
+
+    def __init__(self, eng: Engine, max_targets, max_window, max_K, pool_bytes, digest=True,
+                 compact=False, graph=True, timed=False):
+        self.eng = eng
+        self._h = _P()
+        self.compact = compact
+        self.digest = digest
+        flags = ((EIMP_BATCH_DIGEST if digest else 0) | (EIMP_BATCH_COMPACT if compact else 0)
+                 | (0 if graph else EIMP_STREAM_NO_GRAPH) | (EIMP_STREAM_TIMED if timed else 0))
+        _check(lib().eimp_stream_create(eng._h, max_targets, max_window, max_K, pool_bytes, flags,
+                                        C.byref(self._h)))
+        self.max_targets = max_targets
+        self.timed = timed
+        self._n = 0
+        self._W = []
+
+    def close(self):
+        if self._h:
+            lib().eimp_stream_destroy(self._h)
+            self._h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, gts, s0s, Ws, Ks, ignore=None, cMmaxSplit=0.125):
+        n = len(gts)
+        gts = [np.ascontiguousarray(g, dtype=np.uint64) for g in gts]
+        ts = (EimpTarget * max(n, 1))()
+        for i in range(n):
+            ts[i] = EimpTarget(gts[i].ctypes.data, int(s0s[i]), int(Ws[i]), int(Ks[i]),
+                               -1 if ignore is None else int(ignore[i]))
+        _check(lib().eimp_stream_submit(self._h, n, ts, C.c_float(cMmaxSplit)))
+        self._n = n
+        self._W = [int(w) for w in Ws]
+
+    def done(self) -> bool:
+        rc = lib().eimp_stream_query(self._h)
+        if rc == EIMP_BUSY:
+            return False
+        _check(rc)
+        return True
+
+    def wait(self):
+        """Returns (BatchResult, ndeferred, (kernel_ms, submission_ms))."""
+        n = self._n
+        rs = (EimpResult * max(n, 1))()
+        nd = _SZ()
+        tm = np.zeros(2, np.float64)
+        _check(lib().eimp_stream_wait(self._h, rs, C.byref(nd), _p(tm)))
+        arr = np.frombuffer(rs, dtype=np.uint64).reshape(-1, 6)[:n]
+        self._last = arr.copy()
+        res = BatchResult(S=arr[:, 0].copy(), Mp=arr[:, 1].copy(), ncalls=arr[:, 2].copy(),
+                          K=arr[:, 3].copy(), dig_pbwt=arr[:, 4].copy(),
+                          dig_count0=arr[:, 5].copy())
+        return res, nd.value, (float(tm[0]), float(tm[1]))
+
+    def kernel_ms(self) -> dict:
+        ms = np.zeros(len(KCLASSES), np.float64)
+        _check(lib().eimp_stream_kernel_ms(self._h, _p(ms)))
+        return {k: float(ms[i]) for i, k in enumerate(KCLASSES)}
+
+    def copy_target(self, t, cref=False) -> dict:
+        S, Mp, _, K = (int(x) for x in self._last[t, :4])
+        return _copy_out(lib().eimp_stream_copy_target, self._h, t, S, Mp, K, self._W[t],
+                         self.compact, self.eng.Nad, cref)

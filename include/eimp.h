@@ -147,6 +147,69 @@ int eimp_engine_copy_target(eimp_engine *engine, size_t t, uint32_t *besthaps, u
                             int32_t *count0, uint32_t *pbwt, uint64_t *cref_rows);
 
 /* ---------------------------------------------------------------------- */
+/* Stream contexts: asynchronous, device-driven batches.  No counterpart in  */
+/* the reference, whose GPU calls are synchronous on the default stream     */
+/* (src/GPUEngine.cu:98-217); a context moves the producer/consumer         */
+/* pipeline of src/GPUHandler.cpp:104-186 onto the device.                  */
+/* ---------------------------------------------------------------------- */
+
+typedef struct eimp_stream eimp_stream;
+
+#define EIMP_BUSY 1                /* eimp_stream_query: submission still running */
+#define EIMP_STREAM_NO_GRAPH 8u    /* launch kernels one by one instead of replaying
+                                      the context's captured hipGraph */
+#define EIMP_STREAM_TIMED 16u      /* direct launches with HIP events between the
+                                      kernel classes (eimp_stream_kernel_ms) */
+
+/* Creates a context on an initialized engine (cM set): its own HIP stream,
+ * fixed device buffers for up to max_targets (<= 1024) targets with windows of
+ * up to max_window sites and K up to max_K (after clamping), an output pool of
+ * pool_bytes, and the captured launch sequence.  flags: EIMP_BATCH_DIGEST,
+ * EIMP_BATCH_COMPACT, EIMP_STREAM_NO_GRAPH.  The panel is shared with the
+ * engine and its other contexts; each context is used by one host thread at
+ * a time. */
+int eimp_stream_create(eimp_engine *engine, size_t max_targets, size_t max_window, size_t max_K,
+                       size_t pool_bytes, uint32_t flags, eimp_stream **stream);
+
+/* Queues a batch and returns without waiting.  On the device: the prep
+ * kernels, a planner that sizes each target's outputs from its split count
+ * and places them in the pool, then the cref and PBWT kernels pulling work
+ * from device queues (largest first).  Targets whose outputs do not fit the
+ * pool are deferred.  The caller may reuse targets[].gt when the call
+ * returns.  One submission per context until eimp_stream_wait collects it. */
+int eimp_stream_submit(eimp_stream *stream, size_t n, const eimp_target *targets,
+                       float cMmaxSplit);
+
+/* EIMP_OK when the last submission has completed, EIMP_BUSY while it runs. */
+int eimp_stream_query(eimp_stream *stream);
+
+/* Waits for the last submission and fills results[0..n-1].  A deferred target
+ * has results[t].Mp == 0 (its S is set); *ndeferred counts them.  times: NULL
+ * or 2 doubles, device milliseconds of [0] the kernels and [1] the whole
+ * submission including its copies. */
+int eimp_stream_wait(eimp_stream *stream, eimp_target_result *results, size_t *ndeferred,
+                     double *times);
+
+/* Device milliseconds per kernel class (EIMP_KCLASS_*, EIMP_NKCLASS values)
+ * of the last collected submission of a context created with
+ * EIMP_STREAM_TIMED.  The planner counts as cref, the count0 digest as PBWT;
+ * EIMP_KCLASS_COPY holds the submission's host<->device copies. */
+int eimp_stream_kernel_ms(eimp_stream *stream, double *ms);
+
+/* As eimp_engine_copy_target, for target t of the last collected submission
+ * (outputs stay in the context's pool until its next submission). */
+int eimp_stream_copy_target(eimp_stream *stream, size_t t, uint32_t *besthaps, uint64_t *bhflags,
+                            uint32_t *splitsites, uint64_t *splits, float *cMdiffs,
+                            int32_t *count0, uint32_t *pbwt, uint64_t *cref_rows);
+
+/* Waits for outstanding work and releases the context. */
+int eimp_stream_destroy(eimp_stream *stream);
+
+/* Free and total device memory of the engine's device (admission by free
+ * HBM). */
+int eimp_engine_mem_info(eimp_engine *engine, size_t *free_bytes, size_t *total_bytes);
+
+/* ---------------------------------------------------------------------- */
 /* Measurement                                                              */
 /* ---------------------------------------------------------------------- */
 

@@ -1,0 +1,32 @@
+"""Shared checks of the GPU parity tests (device outputs vs the oracle)."""
+import numpy as np
+
+import oracle
+from eagleimp_amd.engine import Engine
+
+
+def panel_engine(panel, Ncorr=None, haploid=None, give_hap=False):
+    eng = Engine(0)
+    Nad = panel.N
+    Ncorr = Nad if Ncorr is None else Ncorr
+    eng.initialize(min(Ncorr, 2), panel.M, Ncorr, Nad, panel.site)
+    eng.set_panel_extras(panel.cM, hap=panel.hap if give_hap else None, haploid=haploid)
+    return eng
+
+
+def check_target(ref, got, K, compact, what):
+    assert np.array_equal(got["besthaps"], ref["besthaps"]), f"{what}: besthaps"
+    assert np.array_equal(got["bhflags"], ref["flags"]), f"{what}: best-hap flags"
+    assert np.array_equal(got["splitsites"], ref["splitsites"]), f"{what}: split sites"
+    assert np.array_equal(got["splits"], ref["splits"]), f"{what}: split bitmap"
+    assert np.array_equal(got["cMdiffs"].view(np.uint32), ref["cMdiffs"].view(np.uint32)), \
+        f"{what}: cM diffs"
+    assert np.array_equal(got["count0"], ref["count0"]), f"{what}: count0"
+    if "cref" in got:
+        assert np.array_equal(got["cref"], ref["cref"]), f"{what}: condensed reference"
+    KW = oracle.pbwt_row_words(K)
+    if compact:
+        assert np.array_equal(got["pbwt"], ref["pbwt"]), f"{what}: cPBWT (compact)"
+    else:
+        assert np.array_equal(got["pbwt"][:, :KW], ref["pbwt"]), f"{what}: cPBWT logical words"
+        assert not got["pbwt"][:, KW:].any(), f"{what}: cPBWT padding pairs not zero"

@@ -44,7 +44,7 @@ def test_error_paths_without_device():
 def test_variant_table_covers_config_k():
     vs = engine.pbwt_variants()
     assert max(v["max_k"] for v in vs) >= 16384
-    assert any(v["max_k"] >= 10048 and v["threads"] == 256 for v in vs)
+    assert any(v["max_k"] >= 10048 and v["threads"] == 512 for v in vs)
 
 
 @pytest.mark.skipif(engine.device_count() > 0, reason="a device is visible")

@@ -1,0 +1,203 @@
+"""Stream contexts (eimp_stream_*): the device-driven batch path -- planner,
+queue-driven cref/PBWT kernels, captured hipGraph -- vs the CPU oracle, bit
+for bit, through the C ABI.  Also: graph replay == direct launches, deferral
+when the output pool is short, and results of a context alone == the same
+context's results while other contexts run at the same time.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from eagleimp_amd import synth
+from eagleimp_amd.engine import EimpError
+
+from conftest import say
+from parity import check_target, panel_engine
+
+pytestmark = pytest.mark.gpu
+
+MB = 1 << 20
+
+
+@pytest.fixture(scope="module")
+def small():
+    return synth.make_panel(12, N=300, M=4000, F=20, T=24)
+
+
+def _refs(pv, tg, idx):
+    return [oracle.process_target(pv if not callable(pv) else pv(t), tg.gt[t], tg.s0[t], tg.W[t],
+                                  tg.K[t]) for t in idx]
+
+
+def _compare(st, res, refs, idx, compact, tag):
+    for j, t in enumerate(idx):
+        r = refs[j]
+        what = f"{tag} target {j}"
+        assert int(res.S[j]) == r["S"], f"{what}: S"
+        assert int(res.Mp[j]) == r["Mp"], f"{what}: Mp"
+        assert int(res.ncalls[j]) == r["ncalls"], f"{what}: ncalls"
+        got = st.copy_target(j, cref=True)
+        check_target(r, got, int(r["K"]), compact, what)
+        assert int(res.dig_pbwt[j]) == r["dig_pbwt"], f"{what}: pbwt digest"
+        assert int(res.dig_count0[j]) == r["dig_count0"], f"{what}: count0 digest"
+
+
+@pytest.mark.parametrize("compact", [False, True])
+def test_stream_matches_oracle(built, small, compact):
+    """Two submissions in a row on one context (the second smaller), mixed
+    windows and K including the whole-panel identity case."""
+    panel = small
+    pv = oracle.PanelView(panel.hap, panel.site, panel.cM)
+    eng = panel_engine(panel)
+    tg = synth.make_targets(panel, 41, 12, kset=[300, 64, 150, 299, 2, 257], wmin=60, wmax=4000,
+                            miss=0.0, miss_hi=0.05)
+    st = eng.stream(16, 4000, 300, 256 * MB, compact=compact)
+    for idx in (list(range(12)), [3, 7, 1]):
+        say("stream compact", compact, "targets", idx, "K", [int(tg.K[t]) for t in idx])
+        st.submit([tg.gt[t] for t in idx], [tg.s0[t] for t in idx], [tg.W[t] for t in idx],
+                  [tg.K[t] for t in idx])
+        res, nd, (kms, sms) = st.wait()
+        assert nd == 0
+        assert 0 <= kms <= sms
+        _compare(st, res, _refs(pv, tg, idx), idx, compact, f"stream {idx}")
+    st.close()
+    eng.close()
+
+
+def test_stream_haploid_and_ignored(built, small):
+    panel = small
+    Nad = panel.N
+    hapl = np.zeros(Nad // 2, np.uint8)
+    hapl[[2, 9, 40]] = 1
+    Ncorr = Nad - int(hapl.sum()) - 2
+    ignore = [4, 33, 0, 149]
+    eng = panel_engine(panel, Ncorr=Ncorr, haploid=hapl)
+    tg = synth.make_targets(panel, 43, 4, kset=[30, 180, Ncorr], wmin=400, wmax=4000)
+    pvs = [oracle.PanelView(panel.hap, panel.site, panel.cM, haploid_ref=hapl, ignore_sample=ig,
+                            N_corr=Ncorr) for ig in ignore]
+    st = eng.stream(4, 4000, Ncorr, 64 * MB)
+    say("stream haploid", list(tg.K))
+    st.submit(tg.gt, tg.s0, tg.W, tg.K, ignore=ignore)
+    res, nd, _ = st.wait()
+    assert nd == 0
+    _compare(st, res, _refs(lambda t: pvs[t], tg, range(4)), range(4), False, "haploid")
+    st.close()
+    eng.close()
+
+
+def test_graph_replay_equals_direct_launches(built, small):
+    panel = small
+    eng = panel_engine(panel)
+    tg = synth.make_targets(panel, 45, 10, kset=[100, 300, 200], wmin=500, wmax=4000)
+    out = []
+    for mode in (dict(graph=True), dict(graph=False), dict(graph=False, timed=True)):
+        st = eng.stream(10, 4000, 300, 128 * MB, **mode)
+        reps = []
+        for rep in range(3):  # replays with host work between them
+            say("mode", mode, "replay", rep)
+            st.submit(tg.gt, tg.s0, tg.W, tg.K)
+            res, nd, _ = st.wait()
+            assert nd == 0
+            reps.append((res.S.copy(), res.dig_pbwt.copy(), res.dig_count0.copy()))
+            if mode.get("timed"):
+                ms = st.kernel_ms()
+                assert ms["pbwt"] > 0 and ms["cref"] > 0 and ms["select"] > 0
+            else:
+                with pytest.raises(EimpError):
+                    st.kernel_ms()
+        for r in reps[1:]:
+            for a, b in zip(reps[0], r):
+                assert np.array_equal(a, b)
+        out.append(reps[0])
+        st.close()
+    for o in out[1:]:
+        for a, b in zip(out[0], o):
+            assert np.array_equal(a, b)
+    eng.close()
+
+
+def test_short_pool_defers_a_suffix(built, small):
+    """Targets whose outputs do not fit are deferred (Mp = 0) and come out
+    right when submitted again."""
+    panel = small
+    pv = oracle.PanelView(panel.hap, panel.site, panel.cM)
+    eng = panel_engine(panel)
+    tg = synth.make_targets(panel, 47, 8, K=300, wmin=3000, wmax=4000)
+    refs = _refs(pv, tg, range(8))
+
+    def region(r):
+        mp, kpad, rs = r["Mp"], (300 + 63) // 64 * 64, 4 * 8 * ((300 + 511) // 512)
+        up = lambda x: (x + 255) // 256 * 256  # noqa: E731
+        return up((mp + 15) // 16 * kpad * 2) + up(mp * 4) + up(2 * mp * rs * 4)
+
+    need = [region(r) for r in refs]
+    pool = sum(need[:3]) + need[3] // 2  # first three fit, the fourth does not
+    st = eng.stream(8, 4000, 300, pool)
+    say("short pool", pool, need)
+    st.submit(tg.gt, tg.s0, tg.W, tg.K)
+    res, nd, _ = st.wait()
+    assert nd == 5
+    assert list(res.Mp[3:]) == [0] * 5
+    assert [int(s) for s in res.S] == [r["S"] for r in refs]
+    _compare(st, res, refs[:3], range(3), False, "fits")
+    with pytest.raises(EimpError):
+        st.copy_target(4)
+    idx = list(range(3, 6))
+    st.submit([tg.gt[t] for t in idx], [tg.s0[t] for t in idx], [tg.W[t] for t in idx],
+              [tg.K[t] for t in idx])
+    res, nd, _ = st.wait()
+    assert nd == 0
+    _compare(st, res, refs[3:6], idx, False, "resubmitted")
+    st.close()
+    eng.close()
+
+
+def test_contexts_alone_and_in_the_mix(built, small):
+    """Each context's results are the same alone and while three contexts
+    overlap on the device."""
+    panel = small
+    eng = panel_engine(panel)
+    batches = [synth.make_targets(panel, 50 + i, n, kset=[64, 200, 300], wmin=200, wmax=4000)
+               for i, n in enumerate((9, 3, 14))]
+    sts = [eng.stream(16, 4000, 300, 128 * MB) for _ in batches]
+    solo = []
+    for st, tg in zip(sts, batches):
+        st.submit(tg.gt, tg.s0, tg.W, tg.K)
+        res, nd, _ = st.wait()
+        assert nd == 0
+        solo.append(res)
+    for rnd in range(3):
+        say("mix round", rnd)
+        for st, tg in zip(sts, batches):
+            st.submit(tg.gt, tg.s0, tg.W, tg.K)
+        for st, ref in zip(sts, solo):
+            res, nd, _ = st.wait()
+            assert nd == 0
+            for f in ("S", "Mp", "ncalls", "dig_pbwt", "dig_count0"):
+                assert np.array_equal(getattr(res, f), getattr(ref, f)), f
+    for st in sts:
+        st.close()
+    eng.close()
+
+
+def test_stream_argument_errors(built, small):
+    panel = small
+    eng = panel_engine(panel)
+    tg = synth.make_targets(panel, 60, 3, K=100, wmin=100, wmax=2000)
+    with pytest.raises(EimpError):
+        eng.stream(2000, 4000, 300, MB)  # more than 1024 targets
+    st = eng.stream(2, 2000, 100, 16 * MB)
+    with pytest.raises(EimpError):
+        st.submit(tg.gt, tg.s0, tg.W, tg.K)  # 3 > max_targets
+    with pytest.raises(EimpError):
+        st.submit([tg.gt[0]], [0], [3000], [100])  # window beyond max_window
+    with pytest.raises(EimpError):
+        st.submit([tg.gt[0]], [0], [tg.W[0]], [200])  # K beyond max_K
+    with pytest.raises(EimpError):
+        st.wait()  # nothing submitted
+    st.submit([tg.gt[0]], [tg.s0[0]], [tg.W[0]], [100])
+    with pytest.raises(EimpError):
+        st.submit([tg.gt[0]], [tg.s0[0]], [tg.W[0]], [100])  # not collected yet
+    st.wait()
+    st.close()
+    eng.close()
