@@ -81,12 +81,64 @@ def rocpd_summary(dbs):
         print()
 
 
+def serve_bench(args):
+    """Workload 5: the serving loop (eagleimp_amd.serve) with the 20k x 200k
+    panel resident; one run of --requests Poisson arrivals per rate."""
+    from eagleimp_amd import serve, synth
+    from eagleimp_amd.engine import Engine
+
+    cfg, panel, pool = synth.make_workload(5, T=args.targets)
+    log(f"workload {cfg['name']}: panel N={panel.N} M={panel.M}, pool {len(pool.W)} targets")
+    eng = Engine(0)
+    eng.initialize(2, panel.M, panel.N, panel.N, panel.site)
+    eng.set_panel_extras(panel.cM, hap=panel.hap)
+    srv = serve.Server(eng, pool, int(pool.W.max()), int(pool.K.max()), contexts=args.contexts + 1)
+    warm = serve.make_requests([0.0], [64], [0], len(pool.W))
+    serve.run_requests(srv, warm, realtime=False)
+    runs = []
+    total_t = total_wall = 0.0
+    for rate in args.rates:
+        t, size, first = synth.make_arrivals(cfg["panel_seed"] + 2, args.requests, rate,
+                                             len(pool.W), cfg["bmin"], cfg["bmax"])
+        reqs = serve.make_requests(t, size, first, len(pool.W))
+        log(f"rate {rate}/s: {len(reqs)} requests, {int(size.sum())} targets")
+        wall = serve.run_requests(srv, reqs)
+        r = serve.summarize(reqs, wall)
+        r["rate_batches_per_s"] = rate
+        r["offered_targets_per_s"] = round(float(size.sum()) / float(t[-1]), 1)
+        runs.append(r)
+        total_t += r["targets"]
+        total_wall += wall
+        log(json.dumps(r))
+    subs, deferred = srv.submissions, srv.deferred
+    srv.close()
+    eng.close()
+    worst = max(runs, key=lambda r: r["p99_ms"])
+    out = {
+        "metric": METRIC, "value": round(total_t / total_wall, 3), "unit": "targets/s",
+        "n_gpus": 1, "steps": sum(r["requests"] for r in runs), "warmup": 1,
+        "ms_per_step": round(total_wall / sum(r["requests"] for r in runs) * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": cfg["name"], "N": panel.N, "M": panel.M,
+                   "pool_targets": len(pool.W), "contexts": args.contexts + 1,
+                   "rates": list(args.rates), "requests_per_rate": args.requests},
+        "p50_ms": worst["p50_ms"], "p99_ms": worst["p99_ms"],
+        "serving": runs, "submissions": subs, "deferred": deferred,
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", type=int, default=2, choices=[1, 2, 3, 4])
+    ap.add_argument("--workload", type=int, default=2, choices=[1, 2, 3, 4, 5])
+    ap.add_argument("--rates", type=float, nargs="+", default=[20.0, 200.0],
+                    help="workload 5: request arrival rates (batches/s)")
+    ap.add_argument("--requests", type=int, default=120, help="workload 5: requests per rate")
     ap.add_argument("--targets", type=int, default=None, help="targets in the workload")
     ap.add_argument("--batch", type=int, default=None, help="targets per step")
     ap.add_argument("--variant", type=int, default=-1, help="PBWT kernel variant (-1 auto)")
@@ -98,6 +150,9 @@ def main():
                     help="stream contexts (device-driven, queue kernels) or the synchronous "
                          "batch engine")
     ap.add_argument("--contexts", type=int, default=2, help="stream contexts in flight")
+    ap.add_argument("--tune", action=argparse.BooleanOptionalAction, default=True,
+                    help="stream engine: pick the PBWT variant and targets per submission with "
+                         "the autotuner on a sample of the workload first")
     ap.add_argument("--graph-steps", type=int, default=2,
                     help="steps replayed through captured hipGraphs after the timed region")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -127,6 +182,10 @@ def main():
                           "(multi-GPU sharding is by target, see DESIGN.md)"}))
         sys.exit(2)
 
+    if args.workload == 5:
+        serve_bench(args)
+        return
+
     from eagleimp_amd import synth
     from eagleimp_amd.engine import Engine
 
@@ -155,6 +214,7 @@ def main():
                              digest=digest)
 
     graph_tps = None
+    tuned = None
     if args.engine == "batch":
         # synchronous engine: one run_batch call per step
         for w in range(args.warmup):
@@ -172,6 +232,15 @@ def main():
         tim = eng.timing()
         pbwt_ms, pbwt_launches = tim["pbwt"]
     else:
+        if args.tune:
+            from eagleimp_amd.tune import tune
+            nt = min(T, 256)
+            cands = [b for b in (128, 256) if b <= nt] or [nt]
+            tuned = tune(eng, tg, list(range(nt)), batches=cands, use_cache=False, log=log)
+            if not args.batch:
+                batch = tuned["batch"]
+            log(f"autotuner: variant {tuned['variant']} ({tuned['threads']} threads x "
+                f"{tuned['chunks_per_wave']} chunks/wave), {batch} targets per submission")
         free, total = eng.mem_info()
         pool = int(max(free - (24 << 30), 1 << 30) // args.contexts)
         maxW = int(tg.W.max())
@@ -360,6 +429,8 @@ def main():
         "kernel_ms": {k: round(v[0], 3) for k, v in tim.items()},
         "plain_targets_per_s": None if plain_tps is None else round(plain_tps, 3),
         "graph_targets_per_s": None if graph_tps is None else round(graph_tps, 3),
+        "autotuned": None if tuned is None else {
+            k: tuned[k] for k in ("variant", "threads", "chunks_per_wave", "batch")},
         "engine": args.engine,
         "cpu_baseline": cpu,
     }

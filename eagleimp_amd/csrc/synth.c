@@ -24,7 +24,8 @@
 
 #define SYNTH_API __attribute__((visibility("default")))
 
-enum { TAG_FOUNDER = 1, TAG_PANEL = 2, TAG_TARGET = 3, TAG_MISSING = 4, TAG_TPARAM = 5 };
+enum { TAG_FOUNDER = 1, TAG_PANEL = 2, TAG_TARGET = 3, TAG_MISSING = 4, TAG_TPARAM = 5,
+       TAG_ARRIVAL = 6 };
 
 typedef struct { uint64_t s[4]; } rng_t;
 
@@ -306,6 +307,32 @@ SYNTH_API int synth_target_params(uint64_t seed, uint64_t T, uint64_t M, uint64_
         s0[t] = rng_below(&r, M - w + 1);
         K[t] = nk ? kset[rng_below(&r, nk)] : 0;
         miss[t] = miss_lo + rng_u01(&r) * (miss_hi - miss_lo);
+    }
+    return 0;
+}
+
+/* Serving workload: n request batches arriving as a Poisson process of the
+ * given rate (batches/s), sizes log-uniform on [bmin, bmax], each batch a run
+ * of consecutive targets of a pool of npool targets starting at first[i]
+ * (mod npool).  Stream TAG_ARRIVAL, one substream per request. */
+SYNTH_API int synth_arrivals(uint64_t seed, uint64_t n, double rate, uint64_t bmin, uint64_t bmax,
+                             uint64_t npool, double *t, uint32_t *size, uint32_t *first) {
+    if (rate <= 0 || bmin < 1 || bmax < bmin || npool < 1) return -1;
+    double now = 0;
+    for (uint64_t i = 0; i < n; i++) {
+        rng_t r;
+        rng_seed(&r, seed, TAG_ARRIVAL, i);
+        now += rng_exp(&r) / rate;
+        t[i] = now;
+        uint64_t b = bmin;
+        if (bmax > bmin) {
+            double lb = log((double)bmin) + rng_u01(&r) * (log((double)bmax) - log((double)bmin));
+            b = (uint64_t)llround(exp(lb));
+        }
+        if (b < bmin) b = bmin;
+        if (b > bmax) b = bmax;
+        size[i] = (uint32_t)b;
+        first[i] = (uint32_t)rng_below(&r, npool);
     }
     return 0;
 }

@@ -40,6 +40,9 @@ def _lib():
                                             C.c_uint64, P, C.c_uint64, C.c_double, C.c_double,
                                             P, P, P, P]
         lib.synth_target_params.restype = C.c_int
+        lib.synth_arrivals.argtypes = [C.c_uint64, C.c_uint64, C.c_double, C.c_uint64,
+                                       C.c_uint64, C.c_uint64, P, P, P]
+        lib.synth_arrivals.restype = C.c_int
         _LIB = lib
     return _LIB
 
@@ -147,6 +150,19 @@ def make_targets(panel: Panel, seed: int, T: int, *, K=None, kset=None, wmin=Non
     return Targets(s0=s0, W=W, K=Ks, miss=ms, gt=gts)
 
 
+def make_arrivals(seed: int, n: int, rate: float, npool: int, bmin: int = 1, bmax: int = 256):
+    """Serving workload: (arrival seconds, batch sizes, first pool index) of n
+    request batches, a Poisson process of `rate` batches/s with log-uniform
+    sizes on [bmin, bmax]."""
+    t = np.zeros(n, np.float64)
+    size = np.zeros(n, np.uint32)
+    first = np.zeros(n, np.uint32)
+    rc = _lib().synth_arrivals(seed, n, float(rate), bmin, bmax, npool, ptr(t), ptr(size), ptr(first))
+    if rc != 0:
+        raise ValueError("bad arrival parameters")
+    return t, size, first
+
+
 # Benchmark workloads.  Site spacing is 1 site/kb with a linear 1 cM/Mb map
 # for every panel; the seed of each workload is its number.
 WORKLOADS = {
@@ -159,13 +175,20 @@ WORKLOADS = {
             miss=0.0, miss_hi=0.05),
     4: dict(name="cfg4-large-200kx500k-K10000", N=200000, M=500000, F=2000, beta_a=0.3,
             T=10000, K=10000, miss=0.01),
+    # serving: the 20k x 200k panel (panel seed 3) resident, a pool of targets
+    # with the windows / K / missingness of the batched workload, request
+    # batches of 1-256 targets arriving at 20-200 batches/s
+    5: dict(name="cfg5-serving-20kx200k-poisson", panel_seed=3, N=20000, M=200000, F=200,
+            beta_a=0.5, T=1024, kset=[2000, 4000, 6000, 8000, 10000], wmin=20000,
+            wmax=200000, miss=0.0, miss_hi=0.05, rates=(20.0, 200.0), bmin=1, bmax=256),
 }
 
 
 def make_workload(cfg: int, T: int | None = None, nthreads: int = 0):
     c = dict(WORKLOADS[cfg])
     T = c["T"] if T is None else T
-    panel = make_panel(cfg, c["N"], c["M"], c["F"], T, beta_a=c["beta_a"], nthreads=nthreads)
+    panel = make_panel(c.get("panel_seed", cfg), c["N"], c["M"], c["F"], T, beta_a=c["beta_a"],
+                       nthreads=nthreads)
     tg = make_targets(panel, cfg, T, K=c.get("K"), kset=c.get("kset"), wmin=c.get("wmin"),
                       wmax=c.get("wmax"), miss=c["miss"], miss_hi=c.get("miss_hi"))
     return c, panel, tg

@@ -1,0 +1,108 @@
+"""Autotuner for the stream-context path: picks the PBWT kernel variant
+(threads per workgroup x chunks per wave, which fixes the workgroup's LDS and
+so the resident workgroups per CU) and the targets per submission, by timing
+a sample batch of the actual workload through timed contexts (HIP events
+between the kernel classes).  Results are cached per panel shape, K range,
+window range and candidate set.
+
+    from eagleimp_amd.tune import tune
+    best = tune(eng, targets, sample_idx)   # sets eng's PBWT variant
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import time
+
+import numpy as np
+
+from .engine import Engine, pbwt_variants
+
+CACHE = os.environ.get("EIMP_TUNE_CACHE",
+                       os.path.join(os.path.expanduser("~"), ".cache", "eagleimp_amd", "tune.json"))
+
+
+def _key(eng: Engine, Kmax: int, Wmax: int, batches, variants) -> str:
+    sig = json.dumps([eng.Nad, eng.Ncorr, eng.M, Kmax, Wmax, list(batches),
+                      [(v["threads"], v["chunks_per_wave"]) for v in variants]])
+    return hashlib.sha1(sig.encode()).hexdigest()[:16]
+
+
+def _load():
+    try:
+        with open(CACHE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(db):
+    try:
+        os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+        tmp = CACHE + f".{os.getpid()}"
+        with open(tmp, "w") as f:
+            json.dump(db, f, indent=1)
+        os.replace(tmp, CACHE)
+    except OSError:
+        pass
+
+
+def candidates(Kmax: int):
+    """PBWT variants whose capacity holds K rounded up to 64."""
+    kpad = (Kmax + 63) // 64 * 64
+    return [v for v in pbwt_variants() if v["max_k"] >= kpad]
+
+
+def tune(eng: Engine, targets, idx, batches=(128, 256), reserve_bytes: int = 24 << 30,
+         use_cache: bool = True, log=None) -> dict:
+    """Times every (variant, batch) candidate on the sample targets[idx] and
+    sets the fastest variant on the engine.  Returns the choice and the table
+    of per-target device milliseconds."""
+    idx = list(idx)
+    Kmax = int(min(int(np.max(targets.K[idx])), eng.Ncorr))
+    Wmax = int(np.max(targets.W[idx]))
+    vs = candidates(Kmax)
+    if not vs:
+        raise ValueError(f"no PBWT variant holds K = {Kmax}")
+    batches = [b for b in batches if b <= len(idx)] or [len(idx)]
+    key = _key(eng, Kmax, Wmax, batches, vs)
+    db = _load() if use_cache else {}
+    if key in db:
+        best = db[key]
+        eng.set_pbwt_variant(best["variant"])
+        best = dict(best, cached=True)
+        return best
+    free, _ = eng.mem_info()
+    pool = max(free - reserve_bytes, 1 << 28)
+    table = []
+    for v in vs:
+        eng.set_pbwt_variant(v["variant"])
+        for b in batches:
+            st = eng.stream(b, Wmax, Kmax, pool, timed=True, graph=False)
+            sub = idx[:b]
+            t0 = time.perf_counter()
+            st.submit([targets.gt[i] for i in sub], targets.s0[sub], targets.W[sub],
+                      targets.K[sub])
+            res, nd, _ = st.wait()
+            ms = st.kernel_ms()
+            wall = time.perf_counter() - t0
+            st.close()
+            done = int((res.Mp > 0).sum())
+            dev = sum(v_ for k, v_ in ms.items() if k != "copy")
+            row = dict(variant=v["variant"], threads=v["threads"],
+                       chunks_per_wave=v["chunks_per_wave"], batch=b, targets=done,
+                       ms_per_target=dev / max(done, 1), pbwt_ms=ms["pbwt"], cref_ms=ms["cref"],
+                       wall_s=round(wall, 3))
+            table.append(row)
+            if log:
+                log(f"tune: {row}")
+    best = min(table, key=lambda r: r["ms_per_target"])
+    choice = dict(variant=best["variant"], threads=best["threads"],
+                  chunks_per_wave=best["chunks_per_wave"], batch=best["batch"],
+                  ms_per_target=best["ms_per_target"], table=table, cached=False)
+    eng.set_pbwt_variant(choice["variant"])
+    if use_cache:
+        db[key] = {k: v for k, v in choice.items() if k != "cached"}
+        _store(db)
+    return choice

@@ -1,0 +1,77 @@
+"""Autotuner selection and cache logic (eagleimp_amd.tune) with stand-in
+contexts whose timings are known; no device."""
+import numpy as np
+
+from eagleimp_amd import tune
+from eagleimp_amd.engine import BatchResult
+
+
+class _Targets:
+    def __init__(self, n, K):
+        self.gt = [np.zeros(16, np.uint64) for _ in range(n)]
+        self.s0 = np.zeros(n, np.uint64)
+        self.W = np.full(n, 1000, np.uint64)
+        self.K = np.full(n, K, np.uint32)
+
+
+class _Stream:
+    def __init__(self, eng, batch):
+        self.eng, self.batch, self.n = eng, batch, 0
+
+    def submit(self, gts, s0, W, K):
+        self.n = len(gts)
+        self.eng.submits += 1
+
+    def wait(self):
+        a = np.ones((self.n, 6), np.uint64)
+        return BatchResult(*(a[:, i] for i in range(6))), 0, (0.0, 0.0)
+
+    def kernel_ms(self):
+        v = self.eng.variant
+        # variant 3 is fastest per target, and larger batches amortise better
+        per = {2: 2.0, 3: 1.0, 4: 1.5}.get(v, 9.0)
+        return {"distance": 0.0, "select": 0.1, "monos": 0.0, "splits": 0.1,
+                "cref": 0.2 * self.n, "pbwt": per * self.n * (1.0 if self.batch > 128 else 1.2),
+                "copy": 5.0}
+
+    def close(self):
+        pass
+
+
+class _Engine:
+    Nad = Ncorr = 20000
+    M = 200000
+
+    def __init__(self):
+        self.variant = -1
+        self.submits = 0
+
+    def set_pbwt_variant(self, v):
+        self.variant = v
+
+    def mem_info(self):
+        return 200 << 30, 288 << 30
+
+    def stream(self, b, W, K, pool, timed=False, graph=True):
+        assert timed and not graph
+        return _Stream(self, b)
+
+
+def test_candidates_hold_k():
+    for K in (100, 2000, 10000, 16384):
+        for v in tune.candidates(K):
+            assert v["max_k"] >= (K + 63) // 64 * 64
+    assert [v["variant"] for v in tune.candidates(10000)] == [2, 3, 4]
+
+
+def test_tune_picks_fastest_and_caches(tmp_path, monkeypatch):
+    monkeypatch.setattr(tune, "CACHE", str(tmp_path / "tune.json"))
+    eng = _Engine()
+    tg = _Targets(300, 10000)
+    best = tune.tune(eng, tg, range(300), batches=(128, 256))
+    assert (best["variant"], best["batch"]) == (3, 256) and not best["cached"]
+    assert eng.variant == 3
+    assert len(best["table"]) == 3 * 2 and eng.submits == 6
+    eng2 = _Engine()
+    again = tune.tune(eng2, tg, range(300), batches=(128, 256))
+    assert again["cached"] and again["variant"] == 3 and eng2.submits == 0 and eng2.variant == 3
