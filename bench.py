@@ -150,6 +150,8 @@ def main():
                     help="stream contexts (device-driven, queue kernels) or the synchronous "
                          "batch engine")
     ap.add_argument("--contexts", type=int, default=2, help="stream contexts in flight")
+    ap.add_argument("--tune-batches", type=int, nargs="+", default=[256, 384, 512, 1024],
+                    help="targets-per-submission candidates of the autotuner")
     ap.add_argument("--tune", action=argparse.BooleanOptionalAction, default=True,
                     help="stream engine: pick the PBWT variant and targets per submission with "
                          "the autotuner on a sample of the workload first")
@@ -234,18 +236,24 @@ def main():
     else:
         if args.tune:
             from eagleimp_amd.tune import tune
-            nt = min(T, 256)
-            cands = [b for b in (128, 256) if b <= nt] or [nt]
+            nt = min(T, max(args.tune_batches))
+            cands = [b for b in args.tune_batches if b <= nt] or [nt]
             tuned = tune(eng, tg, list(range(nt)), batches=cands, use_cache=False, log=log)
             if not args.batch:
                 batch = tuned["batch"]
             log(f"autotuner: variant {tuned['variant']} ({tuned['threads']} threads x "
                 f"{tuned['chunks_per_wave']} chunks/wave), {batch} targets per submission")
         free, total = eng.mem_info()
-        pool = int(max(free - (24 << 30), 1 << 30) // args.contexts)
+        budget = max(free - (24 << 30), 1 << 30)
+        nctx = args.contexts
+        if tuned is not None and tuned.get("bytes_per_target"):
+            # as many contexts as hold a full submission each
+            per = 1.15 * batch * tuned["bytes_per_target"]
+            nctx = int(max(1, min(args.contexts, budget // per)))
+        pool = int(budget // nctx)
         maxW = int(tg.W.max())
         maxK = int(min(int(tg.K.max()), panel.N))
-        log(f"stream contexts: {args.contexts} x pool {pool / 2**30:.1f} GiB "
+        log(f"stream contexts: {nctx} x pool {pool / 2**30:.1f} GiB "
             f"(device free {free / 2**30:.1f} of {total / 2**30:.1f} GiB)")
 
         def pipeline(sts, steps, first):
@@ -290,12 +298,12 @@ def main():
 
         def contexts(timed):
             return [eng.stream(batch, maxW, maxK, pool, graph=not timed, timed=timed)
-                    for _ in range(args.contexts)]
+                    for _ in range(nctx)]
 
         sts = contexts(timed=True)
         log(f"warmup: {args.warmup} steps")
         pipeline(sts, args.warmup, 0)
-        log(f"timed: {args.steps} steps x {batch} targets ({args.contexts} stream contexts, "
+        log(f"timed: {args.steps} steps x {batch} targets ({nctx} stream contexts, "
             f"events between kernel classes)")
         t_start = time.perf_counter()
         results, kms = pipeline(sts, args.steps, args.warmup)

@@ -47,11 +47,12 @@ hipError_t launch_prep_dev(int cls, const PanelDev &P, const TgtDev *tg, const u
 hipError_t launch_plan(TgtDev *tg, BatchDev *B, const PlanArgs &A, hipStream_t s);
 hipError_t launch_digest_count0_dev(const TgtDev *tg, const uint32_t *d_n, uint32_t nmax,
                                     hipStream_t s);
-size_t cref_lds_bytes(const PanelDev &P);
+size_t cref_lds_bytes(const PanelDev &P, uint32_t maxK);
 hipError_t cref_persist_prepare(const PanelDev &P, uint32_t maxK, int *occ);
+bool cref_ident_ok(const PanelDev &P);
 hipError_t launch_cref_persist(const PanelDev &P, const TgtDev *tg, BatchDev *B,
-                               const uint32_t *blockpref, uint32_t maxK, uint32_t grid,
-                               hipStream_t s);
+                               const uint32_t *blockpref, const uint32_t *blockpref_id,
+                               uint32_t maxK, const uint32_t *grid, hipStream_t s);
 hipError_t pbwt_persist_prepare(int v, size_t lds, int *occ);
 hipError_t launch_pbwt_persist(int v, const TgtDev *tg, BatchDev *B, const PbwtItem *items,
                                uint32_t grid, size_t lds, hipStream_t s);
@@ -317,6 +318,8 @@ int run_build(eimp_engine *e, uint32_t n, bool digest, bool compact) {
         pitems.push_back(PbwtItem{t, 1});
     }
     if (Kmax > cref_max_k()) return fail(EIMP_ERR_UNSUPPORTED, "K beyond the cref kernel limit");
+    if (cref_lds_bytes(e->pd, Kmax) > 160 * 1024)
+        return fail(EIMP_ERR_UNSUPPORTED, "panel too wide for the cref kernel's LDS staging");
     int v = e->choose_pbwt_variant(Kmax_pad);
     if (v < 0) return v;
     const size_t lds = pbwt_lds_bytes(v, Kmax_pad);
@@ -445,7 +448,7 @@ struct eimp_stream {
     // launch configuration
     int pbwt_v = -1;
     size_t pbwt_lds = 0;
-    uint32_t cref_grid = 0, pbwt_grid = 0;
+    uint32_t cref_grid[2] = {0, 0}, pbwt_grid = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -470,6 +473,8 @@ int stream_launch_sequence(eimp_stream *s) {
     A.pool_cap = s->pool_cap;
     A.dig_base = s->d_digest.as<uint64_t>();
     A.blockpref = s->d_blockpref.as<uint32_t>();
+    A.blockpref_id = s->d_blockpref.as<uint32_t>() + s->nmax + 1;
+    A.ident_ok = cref_ident_ok(e->pd) ? 1u : 0u;
     A.pitems = s->d_pitems.as<PbwtItem>();
     A.compact = (s->flags & EIMP_BATCH_COMPACT) ? 1u : 0u;
     A.digest = (s->flags & EIMP_BATCH_DIGEST) ? 1u : 0u;
@@ -485,7 +490,7 @@ int stream_launch_sequence(eimp_stream *s) {
     }
     CHK(mark(EIMP_KCLASS_CREF));
     HIPCHK(launch_plan(s->d_tg(), s->d_batch(), A, s->stream));
-    HIPCHK(launch_cref_persist(e->pd, s->d_tg(), s->d_batch(), s->d_blockpref.as<uint32_t>(), s->Kmax,
+    HIPCHK(launch_cref_persist(e->pd, s->d_tg(), s->d_batch(), A.blockpref, A.blockpref_id, s->Kmax,
                                s->cref_grid, s->stream));
     CHK(mark(EIMP_KCLASS_PBWT));
     HIPCHK(launch_pbwt_persist(s->pbwt_v, s->d_tg(), s->d_batch(), s->d_pitems.as<PbwtItem>(),
@@ -540,7 +545,7 @@ int stream_setup(eimp_stream *s, size_t pool_bytes) {
     CHK(s->d_cMdiffs.ensure(nmax * round_up(s->Wmax + 1, 64) * 4, "cM diffs"));
     CHK(s->d_stats.ensure(nmax * 16, "stats"));
     CHK(s->d_digest.ensure(nmax * 16, "digests"));
-    CHK(s->d_blockpref.ensure((nmax + 1) * 4, "block prefix"));
+    CHK(s->d_blockpref.ensure(2 * (nmax + 1) * 4, "block prefix"));
     CHK(s->d_pitems.ensure(2 * nmax * sizeof(PbwtItem), "pbwt items"));
     CHK(s->d_pool.ensure(pool_bytes, "output pool", false));
     s->pool_cap = pool_bytes;
@@ -548,7 +553,7 @@ int stream_setup(eimp_stream *s, size_t pool_bytes) {
     // kernels and grids
     const uint32_t Kpad = (uint32_t)round_up(s->Kmax, 64);
     if (s->Kmax > cref_max_k()) return fail(EIMP_ERR_UNSUPPORTED, "K beyond the cref kernel limit");
-    if (cref_lds_bytes(e->pd) > 160 * 1024)
+    if (cref_lds_bytes(e->pd, s->Kmax) > 160 * 1024)
         return fail(EIMP_ERR_UNSUPPORTED, "panel too wide for the cref kernel's LDS staging");
     int v = e->choose_pbwt_variant(Kpad);
     if (v < 0) return v;
@@ -557,11 +562,14 @@ int stream_setup(eimp_stream *s, size_t pool_bytes) {
         return fail(EIMP_ERR_UNSUPPORTED, "K beyond the PBWT kernel's slots or 160 KiB of LDS");
     s->pbwt_v = v;
     s->pbwt_lds = lds;
-    int ncu = 0, occ_c = 0, occ_p = 0;
+    int ncu = 0, occ_c[2] = {0, 0}, occ_p = 0;
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, e->dev));
-    HIPCHK(cref_persist_prepare(e->pd, s->Kmax, &occ_c));
+    HIPCHK(cref_persist_prepare(e->pd, s->Kmax, occ_c));
     HIPCHK(pbwt_persist_prepare(v, lds, &occ_p));
-    s->cref_grid = (uint32_t)(std::max(ncu, 1) * std::max(occ_c, 1));
+    s->cref_grid[0] = (uint32_t)(std::max(ncu, 1) * std::max(occ_c[0], 1));
+    // identity targets need K == N_ad
+    const bool ident_possible = cref_ident_ok(e->pd) && s->Kmax == e->Nad && e->Ncorr == e->Nad;
+    s->cref_grid[1] = ident_possible ? (uint32_t)(std::max(ncu, 1) * std::max(occ_c[1], 1)) : 0u;
     s->pbwt_grid = (uint32_t)std::min<uint64_t>(2 * nmax, (uint64_t)std::max(ncu, 1) * std::max(occ_p, 1));
     // fixed per-slot pointers of the target table
     TgtDev *tab = reinterpret_cast<TgtDev *>(s->h_in.as<char>() + s->tab_off);

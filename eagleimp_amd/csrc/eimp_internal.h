@@ -62,13 +62,14 @@ struct TgtDev {
 // Device-side header of a batch run by a stream context (device-driven
 // path: no host synchronisation between the phases).
 struct BatchDev {
-    uint32_t n;         // targets in this submission (written by the host)
-    uint32_t ncref;     // cref blocks queued by k_plan
-    uint32_t npbwt;     // PBWT items queued by k_plan
-    uint32_t nfit;      // targets whose outputs fit the pools
-    uint32_t cref_ctr;  // work-queue counters (zeroed by k_plan)
+    uint32_t n;           // targets in this submission (written by the host)
+    uint32_t ncref;       // cref blocks queued by k_plan: subset (gather) targets
+    uint32_t npbwt;       // PBWT items queued by k_plan
+    uint32_t nfit;        // targets whose outputs fit the pools
+    uint32_t cref_ctr;    // work-queue counters (zeroed by k_plan)
     uint32_t pbwt_ctr;
-    uint32_t pad[2];
+    uint32_t ncref_id;    // cref blocks of whole-panel (identity) targets
+    uint32_t cref_id_ctr;
 };
 
 // cref work item: target t, blocks [b0, b1)
@@ -87,10 +88,13 @@ struct PlanArgs {
     char *pool;
     uint64_t pool_cap;
     uint64_t *dig_base;    // 2 words per target
-    uint32_t *blockpref;   // nmax + 1 prefix sums of cref blocks
-    PbwtItem *pitems;      // 2 * nmax
-    uint32_t compact;      // rows of 2*ceil(K/32) instead of 4*Kwords
+    uint32_t *blockpref;     // nmax + 1 prefix sums of cref blocks (gather targets)
+    uint32_t *blockpref_id;  // nmax + 1 prefix sums of cref blocks (identity targets)
+    PbwtItem *pitems;        // 2 * nmax
+    uint32_t compact;        // rows of 2*ceil(K/32) instead of 4*Kwords
     uint32_t digest;
+    uint32_t ident_ok;       // identity targets take the transpose kernel
+    uint32_t pad;
 };
 
 // Bytes of a target's region in a stream context's pool (each part rounded

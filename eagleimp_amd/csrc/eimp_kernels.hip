@@ -249,12 +249,14 @@ __global__ __launch_bounds__(SEL_NT) void k_select(PanelDev P, const TgtDev *__r
             if (tie && trank < need) sel = true;
             ties_before += tcount;
         }
-        uint64_t sm = __ballot(sel);
+        uint32_t scount;
+        const uint32_t pos = sel_before + block_excl_scan(sel ? 1u : 0u, s_w, scount);
+        // exactly K when Ncorr is consistent; the bitmap always matches bh
+        sel = sel && pos < T.K;
+        if (sel) T.bh[pos] = h;
+        const uint64_t sm = __ballot(sel);
         const uint32_t word = base / 64 + wave;
         if (l == 0 && word < nwords) T.bhflags[word] = sm;
-        uint32_t scount;
-        uint32_t pos = sel_before + block_excl_scan(sel ? 1u : 0u, s_w, scount);
-        if (sel && pos < T.K) T.bh[pos] = h;  // exactly K when Ncorr is consistent
         sel_before += scount;
     }
     if (tid == 0) T.stats[3] = sel_before;
@@ -518,50 +520,59 @@ __device__ __forceinline__ void cref_fill(const PanelDev &P, const TgtDev &T, ui
     }
 }
 
-// Block b of a target, K chosen haplotypes by index (any target).  EPT
-// haplotypes per thread.
-template <int EPT>
+// Block b of a target, K chosen haplotypes by index (any target).
+// Haplotype-outer: thread t takes haplotypes k = t, t + CREF_NT, ... and reads
+// the staged rows at bit bh[k] (consecutive lanes read nearby words: bh is
+// ascending), so one register accumulates a haplotype's 16 row bits.  A block
+// whose segments take several fills keeps the partial values in vacc (LDS).
+// count0 from popcounts of each staged row under the best-hap bitmap.
 __device__ __forceinline__ void cref_block_gather(const PanelDev &P, const TgtDev &T, uint32_t b,
-                                                  uint64_t *srow, uint32_t *cnt) {
+                                                  uint64_t *srow, uint32_t *cnt, uint16_t *vacc) {
     const uint32_t tid = threadIdx.x, l = lane_id();
     const uint32_t K = T.K, S = T.S, Kpad = T.Kpad, Mp = T.Mp;
     const uint32_t nw = (P.N_ad + 63) / 64;
+    const uint32_t nw32 = 2 * nw;
     const uint32_t G = cref_group(nw);
     const bool ident = (T.flags & TF_IDENTITY) != 0;
-    uint32_t idx[EPT], V[EPT];
-#pragma unroll
-    for (int e = 0; e < EPT; e++) {
-        const uint32_t k = tid + CREF_NT * e;
-        idx[e] = k < K ? (ident ? k : T.bh[k]) : 0u;
-        V[e] = 0;
-    }
+    const uint64_t *__restrict__ bhf = T.bhflags;
     if (tid < 16) cnt[tid] = 0;
+    uint16_t *__restrict__ dst = T.cref + (uint64_t)b * Kpad;
     const uint32_t send = min(8 * b + 8, S + 1);
     for (uint32_t sa = 8 * b; sa < send; sa += G) {
         const uint32_t sb = min(sa + G, send);
         cref_fill(P, T, sa, sb, nw, srow);
         __syncthreads();
         const uint32_t nrows = 2 * (sb - sa) - (sb == S + 1 ? 1u : 0u);
+        const uint32_t rb0 = 2 * (sa - 8 * b);
+        // ones among the chosen haplotypes, per row
         for (uint32_t rr = 0; rr < nrows; rr++) {
-            const uint32_t rb = 2 * (sa - 8 * b) + rr;
             const uint64_t *row = srow + (uint64_t)rr * nw;
             uint32_t ones = 0;
+            for (uint32_t w = tid; w < nw; w += CREF_NT) ones += (uint32_t)__popcll(row[w] & bhf[w]);
+            ones = wave_sum(ones);
+            if (l == 0 && ones) atomicAdd(&cnt[rb0 + rr], ones);
+        }
+        const bool first = sa == 8 * b, last = sb == send;
+        const uint32_t *rows32 = (const uint32_t *)srow;
+        for (uint32_t k = tid; k < Kpad; k += CREF_NT) {
+            uint32_t v = 0xFFFFu;  // haplotypes K..Kpad-1: the PBWT kernel's padding
+            if (k < K) {
+                const uint32_t idx = ident ? k : T.bh[k];
+                const uint32_t *col = rows32 + (idx >> 5);
+                const uint32_t sh = idx & 31;
+                v = first ? 0u : (uint32_t)vacc[k];
+                if (nrows == 16) {
 #pragma unroll
-            for (int e = 0; e < EPT; e++) {
-                const uint32_t k = tid + CREF_NT * e;
-                const uint32_t bit = k < K ? (uint32_t)((row[idx[e] >> 6] >> (idx[e] & 63)) & 1ull) : 0u;
-                V[e] |= bit << rb;
-                ones += (uint32_t)__popcll(__ballot(bit));
+                    for (uint32_t rr = 0; rr < 16; rr++) v |= ((col[rr * nw32] >> sh) & 1u) << rr;
+                } else {
+                    for (uint32_t rr = 0; rr < nrows; rr++)
+                        v |= ((col[rr * nw32] >> sh) & 1u) << (rb0 + rr);
+                }
             }
-            if (l == 0) atomicAdd(&cnt[rb], ones);
+            if (last) dst[k] = (uint16_t)v;
+            else vacc[k] = (uint16_t)v;
         }
         __syncthreads();
-    }
-    uint16_t *__restrict__ dst = T.cref + (uint64_t)b * Kpad;
-#pragma unroll
-    for (int e = 0; e < EPT; e++) {
-        const uint32_t k = tid + CREF_NT * e;
-        if (k < Kpad) dst[k] = k < K ? (uint16_t)V[e] : (uint16_t)0xFFFFu;
     }
     if (tid < 16) {
         const uint32_t r = 16 * b + tid;
@@ -641,13 +652,13 @@ __device__ __forceinline__ void cref_block_ident(const PanelDev &P, const TgtDev
     __syncthreads();
 }
 
-template <int EPT>
 __global__ __launch_bounds__(CREF_NT) void k_cref(PanelDev P, const TgtDev *__restrict__ tg,
                                                  const CrefItem *__restrict__ items) {
     extern __shared__ uint64_t srow[];
     __shared__ uint32_t cnt[16];
     const CrefItem it = items[blockIdx.x];
-    for (uint32_t b = it.b0; b < it.b1; b++) cref_block_gather<EPT>(P, tg[it.t], b, srow, cnt);
+    uint16_t *vacc = (uint16_t *)(srow + 2 * cref_group((P.N_ad + 63) / 64) * ((P.N_ad + 63) / 64));
+    for (uint32_t b = it.b0; b < it.b1; b++) cref_block_gather(P, tg[it.t], b, srow, cnt, vacc);
 }
 
 template <int GPT>
@@ -662,20 +673,22 @@ __global__ __launch_bounds__(CREF_NT) void k_cref_ident(PanelDev P, const TgtDev
 // ---------------------------------------------------------------------------
 // a9: compressed PBWT.  One workgroup per (target, direction).
 //
-// The K positions of the current order are cut into chunks of 64; chunk ch is
-// held by wave ch % NW in slot ch / NW (so any K up to the variant's capacity
-// keeps all waves busy), one position per lane, as a 32-bit payload: hap
-// index << 16 | the 16 bits of the current cref block for that haplotype.
-// Positions K..Kpad-1 are padding whose row bits are all ones: under the
+// The K positions of the current order are cut into chunks of 64; wave w
+// holds chunks [w*NS, w*NS+NS) (NS slots, chosen per target so that all waves
+// are busy), one position per lane, as a 32-bit payload: hap index << 16 |
+// the 16 bits of the current cref block for that haplotype.  Positions
+// K..Kpad-1 (and whole padding chunks) carry all-ones row bits: under the
 // stable partition (zeros first) they never move, and they give the
 // 1-padding of the last word.  A step:
 //   pass 1  ballot of the row bit per slot: the slot's two data words go to
-//           output lane c; zero counts per chunk to LDS
-//   (barrier) each lane reads one row of chunk counts: zeros before every
-//           chunk by a wave scan, total zeros (count0)
-//   pass 2  destination of each payload under the partition from mbcnt of
-//           the ballot; payloads scattered into LDS (two contiguous runs per
-//           chunk); output lanes store their data / cumulative-zero pairs
+//           output lane c; the wave's zero count (scalar) to LDS
+//   (barrier) zeros before the wave's first chunk and in the row (count0)
+//           from the NW wave counts, in scalar registers
+//   pass 2  per slot: zeros / ones before the chunk are scalar running sums,
+//           so mbcnt of the ballot (or its complement) with that base is the
+//           destination under the partition; payloads scattered into LDS (two
+//           contiguous runs per chunk); output lanes store their data /
+//           cumulative-zero pairs
 //   (barrier)
 //   pass 3  read back the payloads of my positions.
 // At each 16-row block boundary the next block's uint16 values are staged in
@@ -702,8 +715,25 @@ __device__ __forceinline__ void static_for(F &&f) {
     }
 }
 
-// NS = slots per wave for this target: chunks NS*NW > nch are padding
-// (all-ones bits, never moving), so every slot is processed unconditionally.
+// Exclusive prefix sum of v over lanes 0..N-1 (other lanes: garbage), by DPP
+// row shifts and row broadcasts (no LDS).
+template <int N>
+__device__ __forceinline__ uint32_t lane_scan_excl(uint32_t v) {
+    uint32_t x = v;
+    // within rows of 16 lanes: lane i += lane i-1, i-2, i-4, i-8 (zero past the row start)
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, true);
+    if constexpr (N > 2) x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, true);
+    if constexpr (N > 4) x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, true);
+    if constexpr (N > 8) x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true);
+    // across rows: lane 15 of each row into rows 1 and 3, then lane 31 into rows 2 and 3
+    if constexpr (N > 16) x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);
+    if constexpr (N > 32) x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);
+    return x - v;
+}
+
+// NS = slots per wave for this target.  Wave w owns chunks [w*NS, w*NS+NS);
+// chunks >= Kpad/64 are padding (all-ones bits, never moving), so every slot
+// is processed unconditionally.
 template <int NT, int NS>
 __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint32_t *smem) {
     constexpr int NW = NT / 64;
@@ -713,21 +743,22 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
     const uint32_t nch = Kpad / 64;
     const uint32_t npairs = (K + 31) / 32;
     const uint32_t KW = 2 * npairs;
-    uint32_t *pay = smem;                                    // NPOS
-    uint16_t *stage = (uint16_t *)(smem + NPOS);             // NPOS
-    uint32_t *zarr = smem + NPOS + NPOS / 2;                 // 2 x NS x NW chunk zero counts
+    uint32_t *pay = smem;                          // NPOS payloads
+    // Kpad uint16: the next cref block by haplotype, staged at block boundaries
+    // (when no payloads are in flight) in the first half of pay[]
+    uint16_t *stage = (uint16_t *)smem;
+    uint32_t *wzero = smem + NPOS;                 // NW: zeros of each wave's chunks
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = lane_id();
+    const uint32_t ch0 = wave * NS;                // first chunk of this wave
     const uint16_t *__restrict__ cref = T.cref;
     const bool dig = T.digest != nullptr;
 
-    // haplotypes at positions >= Kpad read all-ones rows from the stage
-    for (uint32_t q = Kpad + threadIdx.x; q < NPOS; q += NT) stage[q] = 0xFFFFu;
     uint32_t p[NS];
     {
         const uint32_t b0 = (rev ? Mp - 1 : 0) >> 4;
 #pragma unroll
         for (int c = 0; c < NS; c++) {
-            const uint32_t pos = ((uint32_t)c * NW + wave) * 64 + ln;
+            const uint32_t pos = (ch0 + (uint32_t)c) * 64 + ln;
             // the cref kernel writes all-ones for K <= k < Kpad
             const uint32_t v = pos < Kpad ? (uint32_t)cref[(uint64_t)b0 * Kpad + pos] : 0xFFFFu;
             p[c] = (pos << 16) | v;
@@ -737,9 +768,9 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
     const uint64_t gbase = rev ? (uint64_t)Mp * KW : 0ull;
     uint64_t dsum = 0;
     const uint32_t nslots = RS / 4;        // 4-word slots of a stored row
-    const uint32_t och = ln * NW + wave;   // chunk stored by this lane
+    const uint32_t och = ch0 + ln;         // chunk stored by this lane
     const bool ostore = ln < (uint32_t)NS && och < nch;
-    const uint32_t ln4 = 4 * ln;
+    const uint32_t ln4 = 4u * ln;
 
     for (uint32_t step = 0; step < Mp; ++step) {
         const uint32_t r = rev ? Mp - 1 - step : step;
@@ -747,60 +778,59 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
         if (step > 0 && (rev ? j == 15 : j == 0)) {
             const uint32_t *__restrict__ src = (const uint32_t *)(cref + (uint64_t)(r >> 4) * Kpad);
             uint32_t *st32 = (uint32_t *)stage;
+            __syncthreads();  // every wave has read back its payloads
             for (uint32_t q = threadIdx.x; q < Kpad / 2; q += NT) st32[q] = src[q];
             __syncthreads();
 #pragma unroll
             for (int c = 0; c < NS; c++) {
+                // haplotypes at positions >= Kpad (padding chunks) keep all-ones rows
                 const uint32_t id = p[c] >> 16;
-                p[c] = (id << 16) | (uint32_t)stage[id];
+                p[c] = (id << 16) | (id < Kpad ? (uint32_t)stage[id] : 0xFFFFu);
             }
         }
-        // pass 1: slot c's ballot words go to lane c
-        uint32_t olo = 0, ohi = 0;
+        // pass 1: slot c's ballot words go to lane c; zeros of the wave's chunks
+        uint32_t olo = 0, ohi = 0, wz = 0;
+        uint32_t bit[NS];
         static_for<0, NS>([&](auto ci) {
             constexpr int c = decltype(ci)::value;
-            const uint64_t m = __ballot((p[c] >> j) & 1u);
+            bit[c] = (p[c] >> j) & 1u;
+            const uint64_t m = __ballot(bit[c]);
             olo = writelane<c>(olo, (uint32_t)m);
             ohi = writelane<c>(ohi, (uint32_t)(m >> 32));
+            wz += 64u - (uint32_t)__popcll(m);
         });
-        const uint32_t zc = ln < (uint32_t)NS ? 64u - (uint32_t)__popc(olo) - (uint32_t)__popc(ohi) : 0u;
-        uint32_t *zp = zarr + (step & 1u) * (NS * NW);
-        if (ln < (uint32_t)NS) zp[ln * NW + wave] = zc;
+        if (ln == 0) wzero[wave] = wz;
         __syncthreads();
-        // lane c: zeros in slot row c (all waves) and in the waves before mine
-        uint32_t rowtot = 0, inrow = 0;
-        if (ln < (uint32_t)NS) {
+        // zeros before my first chunk, and in the whole row (count0)
+        uint32_t run = 0, cnt0 = 0;
 #pragma unroll
-            for (int w2 = 0; w2 < NW; w2++) {
-                const uint32_t v = zp[ln * NW + w2];
-                rowtot += v;
-                inrow += (w2 < (int)wave) ? v : 0u;
-            }
+        for (int w2 = 0; w2 < NW; w2++) {
+            const uint32_t v = __builtin_amdgcn_readfirstlane(wzero[w2]);
+            cnt0 += v;
+            run += (uint32_t)w2 < wave ? v : 0u;
         }
-        const uint32_t rowinc = wave_scan_incl(rowtot);
-        const uint32_t cnt0 = __builtin_amdgcn_readlane(rowinc, 63);
-        const uint32_t zbv = rowinc - rowtot + inrow;  // zeros before chunk och
+        // lane c: zeros before chunk ch0 + c (lanes >= NS: don't care)
+        const uint32_t zbv = run + lane_scan_excl<NS>(64u - (uint32_t)__popc(olo) - (uint32_t)__popc(ohi));
         // pass 2: stable partition, zeros first; byte addresses into pay[]
         static_for<0, NS>([&](auto ci) {
             constexpr int c = decltype(ci)::value;
-            const uint32_t zb = __builtin_amdgcn_readlane(zbv, c);
-            const uint32_t ch = (uint32_t)c * NW + wave;
-            const uint32_t zb4 = 4 * zb, ob4 = 4 * (cnt0 + ch * 64 - zb);
-            uint32_t b = (p[c] >> j) & 1u;
-            // recompute the ballot here rather than keep NS masks live across
-            // the barrier (they would spill)
+            uint32_t b = bit[c];
+            // the ballot again rather than NS masks live across the barrier
             asm volatile("" : "+v"(b));
             const uint64_t m = __ballot(b != 0);
-            const uint32_t t1 = mbcnt(m);
-            const uint32_t zd4 = zb4 + ln4 - 4 * t1;
-            const uint32_t od4 = ob4 + 4 * t1;
+            const uint32_t zb4 = 4u * run;
+            const uint32_t ob4 = 4u * (cnt0 + (ch0 + (uint32_t)c) * 64u - run);
+            const uint32_t t4 = 4u * mbcnt(m);          // ones below me, in bytes
+            const uint32_t od4 = ob4 + t4;
+            const uint32_t zd4 = (zb4 + ln4) - t4;      // zeros below me = ln - ones
             // unpredictable: keep a v_cndmask (a plain select may become a branch)
             const uint32_t dest4 = __builtin_unpredictable(b != 0) ? od4 : zd4;
             *(uint32_t *)((char *)pay + dest4) = p[c];
+            run += 64u - (uint32_t)__popcll(m);
         });
         __syncthreads();
 #pragma unroll
-        for (int c = 0; c < NS; c++) p[c] = pay[((uint32_t)c * NW + wave) * 64 + ln];
+        for (int c = 0; c < NS; c++) p[c] = pay[(ch0 + (uint32_t)c) * 64 + ln];
         // the row: chunk ch holds pairs 2ch and 2ch+1
         uint32_t *__restrict__ row = outbase + (uint64_t)step * RS;
         if (ostore) {
@@ -939,17 +969,26 @@ __global__ __launch_bounds__(PLAN_NT) void k_plan(TgtDev *__restrict__ tg, Batch
             T.digest = nullptr;
         }
     }
-    uint64_t nb_tot;
-    const uint64_t bp = block_excl_scan64(fit ? nblk : 0u, s_w, nb_tot);
-    if (valid) A.blockpref[t] = (uint32_t)bp;
+    // two cref queues: whole-panel targets (transpose kernel) and the others
+    const bool idk = fit && A.ident_ok && (tg[t].flags & TF_IDENTITY);
+    uint64_t nb_tot, nb_id;
+    const uint64_t bp = block_excl_scan64(fit && !idk ? nblk : 0u, s_w, nb_tot);
+    const uint64_t bpi = block_excl_scan64(idk ? nblk : 0u, s_w, nb_id);
+    if (valid) {
+        A.blockpref[t] = (uint32_t)bp;
+        A.blockpref_id[t] = (uint32_t)bpi;
+    }
     uint64_t nfit;
     (void)block_excl_scan64(fit ? 1u : 0u, s_w, nfit);
     if (t == 0) {
         A.blockpref[n] = (uint32_t)nb_tot;
+        A.blockpref_id[n] = (uint32_t)nb_id;
         B->ncref = (uint32_t)nb_tot;
+        B->ncref_id = (uint32_t)nb_id;
         B->npbwt = 2 * (uint32_t)nfit;
         B->nfit = (uint32_t)nfit;
         B->cref_ctr = 0;
+        B->cref_id_ctr = 0;
         B->pbwt_ctr = 0;
     }
     // PBWT items 2t (forward) and 2t+1 (backward), most work first:
@@ -981,39 +1020,62 @@ __global__ __launch_bounds__(PLAN_NT) void k_plan(TgtDev *__restrict__ tg, Batch
     }
 }
 
-// cref blocks from the queue; both target kinds (identity / subset)
-template <int EPT, int GPT>
+// next cref block from a queue: (block index i, its target t); i >= nq ends
+__device__ __forceinline__ void cref_pop(uint32_t *ctr, uint32_t nq, uint32_t n,
+                                         const uint32_t *__restrict__ pref, uint32_t &i,
+                                         uint32_t &t) {
+    __shared__ uint32_t s_i, s_t;
+    if (threadIdx.x == 0) {
+        const uint32_t ii = atomicAdd(ctr, 1u);
+        uint32_t tt = 0;
+        if (ii < nq) {  // last target whose first block is <= ii (it owns block ii)
+            uint32_t lo = 0, hi = n;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (pref[mid] <= ii) lo = mid; else hi = mid;
+            }
+            tt = lo;
+        }
+        s_i = ii;
+        s_t = tt;
+    }
+    __syncthreads();
+    i = s_i;
+    t = s_t;
+    __syncthreads();
+}
+
+// cref blocks of subset targets from their queue
 __global__ __launch_bounds__(CREF_NT) void k_cref_persist(PanelDev P, const TgtDev *__restrict__ tg,
                                                          BatchDev *__restrict__ B,
-                                                         const uint32_t *__restrict__ blockpref,
-                                                         int ident_ok) {
+                                                         const uint32_t *__restrict__ blockpref) {
     extern __shared__ uint64_t srow[];
     __shared__ uint32_t cnt[16];
-    __shared__ uint32_t s_i, s_t;
-    const uint32_t ncref = B->ncref, n = B->n;
+    const uint32_t nq = B->ncref, n = B->n;
+    const uint32_t nw = (P.N_ad + 63) / 64;
+    uint16_t *vacc = (uint16_t *)(srow + 2 * cref_group(nw) * nw);
     for (;;) {
-        if (threadIdx.x == 0) {
-            const uint32_t i = atomicAdd(&B->cref_ctr, 1u);
-            uint32_t t = 0;
-            if (i < ncref) {  // last target whose first block is <= i
-                uint32_t lo = 0, hi = n;
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (blockpref[mid] <= i) lo = mid; else hi = mid;
-                }
-                t = lo;
-            }
-            s_i = i;
-            s_t = t;
-        }
-        __syncthreads();
-        const uint32_t i = s_i, t = s_t;
-        __syncthreads();
-        if (i >= ncref) break;
-        const TgtDev &T = tg[t];
-        const uint32_t b = i - blockpref[t];
-        if (ident_ok && (T.flags & TF_IDENTITY)) cref_block_ident<GPT>(P, T, b, srow, cnt);
-        else cref_block_gather<EPT>(P, T, b, srow, cnt);
+        uint32_t i, t;
+        cref_pop(&B->cref_ctr, nq, n, blockpref, i, t);
+        if (i >= nq) break;
+        cref_block_gather(P, tg[t], i - blockpref[t], srow, cnt, vacc);
+    }
+}
+
+// cref blocks of whole-panel targets from their queue
+template <int GPT>
+__global__ __launch_bounds__(CREF_NT, 4) void k_cref_persist_ident(PanelDev P,
+                                                               const TgtDev *__restrict__ tg,
+                                                               BatchDev *__restrict__ B,
+                                                               const uint32_t *__restrict__ blockpref) {
+    extern __shared__ uint64_t srow[];
+    __shared__ uint32_t cnt[16];
+    const uint32_t nq = B->ncref_id, n = B->n;
+    for (;;) {
+        uint32_t i, t;
+        cref_pop(&B->cref_id_ctr, nq, n, blockpref, i, t);
+        if (i >= nq) break;
+        cref_block_ident<GPT>(P, tg[t], i - blockpref[t], srow, cnt);
     }
 }
 
@@ -1122,18 +1184,21 @@ hipError_t launch_bits_to_list(const TgtDev *tg, uint32_t n, uint32_t nbits_bh, 
     return hipGetLastError();
 }
 
-#define CREF_CASE(E)                                                                      \
-    if (ept <= E) {                                                                       \
-        hipError_t e_ = raise_lds_limit((const void *)k_cref<E>, lds);                    \
-        if (e_ != hipSuccess) return e_;                                                  \
-        hipLaunchKernelGGL(k_cref<E>, dim3(nitems), dim3(CREF_NT), lds, s, P, tg, items); \
-        return hipGetLastError();                                                         \
-    }
+// dynamic LDS of the cref kernels: the staged rows of one fill, plus the
+// partial values of a block that takes several fills (K up to maxK)
+size_t cref_lds_bytes(const PanelDev &P, uint32_t maxK) {
+    const uint32_t nw = (P.N_ad + 63) / 64;
+    const uint32_t G = cref_group(nw);
+    size_t lds = (size_t)2 * G * nw * sizeof(uint64_t);
+    if (G < 8) lds += ((size_t)maxK + 63) / 64 * 64 * sizeof(uint16_t);
+    return lds;
+}
+
 hipError_t launch_cref(const PanelDev &P, const TgtDev *tg, const CrefItem *items, uint32_t nitems,
                        uint32_t maxK, bool ident, hipStream_t s) {
     if (!nitems) return hipSuccess;
     const uint32_t nw = (P.N_ad + 63) / 64;
-    const size_t lds = (size_t)2 * cref_group(nw) * nw * sizeof(uint64_t);
+    const size_t lds = cref_lds_bytes(P, maxK);
     if (ident) {
         const uint32_t gpt = (nw + CREF_NT - 1) / CREF_NT;
         if (gpt <= 2) {
@@ -1147,19 +1212,14 @@ hipError_t launch_cref(const PanelDev &P, const TgtDev *tg, const CrefItem *item
             return hipGetLastError();
         }
     }
-    const uint32_t ept = (maxK + CREF_NT - 1) / CREF_NT;
-    CREF_CASE(4)
-    CREF_CASE(8)
-    CREF_CASE(16)
-    CREF_CASE(32)
-    CREF_CASE(48)
-    CREF_CASE(64)
-    CREF_CASE(96)
-    return hipErrorInvalidValue;
+    hipError_t e = raise_lds_limit((const void *)k_cref, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cref, dim3(nitems), dim3(CREF_NT), lds, s, P, tg, items);
+    return hipGetLastError();
 }
-#undef CREF_CASE
 
-uint32_t cref_max_k() { return 96 * CREF_NT; }
+// haplotype positions are 16-bit in the PBWT payload
+uint32_t cref_max_k() { return 65536 - 64; }
 
 hipError_t launch_digest_count0(const TgtDev *tg, uint32_t n, hipStream_t s) {
     if (!n) return hipSuccess;
@@ -1197,7 +1257,7 @@ size_t pbwt_lds_bytes(int v, uint32_t Kpad) {
     const int ns = pbwt_slots_for(Kpad, nt);
     if (ns < 0 || ns > kPbwtVariants[v].cpw) return ~(size_t)0;
     const size_t npos = (size_t)ns * nt;  // NS x NW x 64
-    return npos * 4 + npos * 2 + 2ull * ns * (nt / 64) * 4;
+    return npos * 4 + (nt / 64) * 4;      // payloads (the cref stage shares them), wave counts
 }
 hipError_t pbwt_prepare(int v, size_t lds) {
     return raise_lds_limit((const void *)kPbwtVariants[v].fn, lds);
@@ -1245,48 +1305,43 @@ hipError_t launch_digest_count0_dev(const TgtDev *tg, const uint32_t *d_n, uint3
     return hipGetLastError();
 }
 
-using CrefPersistFn = void (*)(PanelDev, const TgtDev *, BatchDev *, const uint32_t *, int);
-#define CREF_P_CASE(E)                                                                  \
-    if (ept <= E) return gpt <= 1 ? k_cref_persist<E, 1> : k_cref_persist<E, 2>;
-static CrefPersistFn cref_persist_fn(uint32_t maxK, uint32_t gpt) {
-    const uint32_t ept = (maxK + CREF_NT - 1) / CREF_NT;
-    CREF_P_CASE(4)
-    CREF_P_CASE(8)
-    CREF_P_CASE(16)
-    CREF_P_CASE(32)
-    CREF_P_CASE(48)
-    CREF_P_CASE(64)
-    CREF_P_CASE(96)
-    return nullptr;
+using CrefPersistFn = void (*)(PanelDev, const TgtDev *, BatchDev *, const uint32_t *);
+static CrefPersistFn cref_persist_fn(const PanelDev &P, bool ident) {
+    if (!ident) return k_cref_persist;
+    const uint32_t gpt = ((P.N_ad + 63) / 64 + CREF_NT - 1) / CREF_NT;
+    return gpt <= 1 ? k_cref_persist_ident<1> : k_cref_persist_ident<2>;
 }
-#undef CREF_P_CASE
 
-size_t cref_lds_bytes(const PanelDev &P) {
-    const uint32_t nw = (P.N_ad + 63) / 64;
-    return (size_t)2 * cref_group(nw) * nw * sizeof(uint64_t);
-}
 // identity targets take the transpose path when every 64-haplotype group
 // fits two per thread
 bool cref_ident_ok(const PanelDev &P) { return (P.N_ad + 63) / 64 <= 2u * CREF_NT; }
 
-// sets the dynamic LDS limit; *occ = resident workgroups per CU
+// sets the dynamic LDS limit; occ[0], occ[1] = resident workgroups per CU of
+// the subset and identity kernels
 hipError_t cref_persist_prepare(const PanelDev &P, uint32_t maxK, int *occ) {
-    const uint32_t nw = (P.N_ad + 63) / 64;
-    CrefPersistFn fn = cref_persist_fn(maxK, (nw + CREF_NT - 1) / CREF_NT);
-    if (!fn) return hipErrorInvalidValue;
-    const size_t lds = cref_lds_bytes(P);
-    hipError_t e = raise_lds_limit((const void *)fn, lds);
-    if (e != hipSuccess) return e;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, (const void *)fn, CREF_NT, lds);
+    const size_t lds = cref_lds_bytes(P, maxK);
+    for (int k = 0; k < 2; k++) {
+        occ[k] = 0;
+        if (k == 1 && !cref_ident_ok(P)) continue;
+        const void *fn = (const void *)cref_persist_fn(P, k == 1);
+        hipError_t e = raise_lds_limit(fn, lds);
+        if (e != hipSuccess) return e;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ[k], fn, CREF_NT, lds);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
+// grid[0], grid[1]: workgroups of the subset and identity kernels (0: none)
 hipError_t launch_cref_persist(const PanelDev &P, const TgtDev *tg, BatchDev *B,
-                               const uint32_t *blockpref, uint32_t maxK, uint32_t grid,
-                               hipStream_t s) {
-    const uint32_t nw = (P.N_ad + 63) / 64;
-    CrefPersistFn fn = cref_persist_fn(maxK, (nw + CREF_NT - 1) / CREF_NT);
-    if (!fn) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(CREF_NT), cref_lds_bytes(P), s, P, tg, B, blockpref,
-                       cref_ident_ok(P) ? 1 : 0);
+                               const uint32_t *blockpref, const uint32_t *blockpref_id,
+                               uint32_t maxK, const uint32_t *grid, hipStream_t s) {
+    const size_t lds = cref_lds_bytes(P, maxK);
+    if (grid[0])
+        hipLaunchKernelGGL(cref_persist_fn(P, false), dim3(grid[0]), dim3(CREF_NT), lds, s, P, tg,
+                           B, blockpref);
+    if (grid[1] && cref_ident_ok(P))
+        hipLaunchKernelGGL(cref_persist_fn(P, true), dim3(grid[1]), dim3(CREF_NT), lds, s, P, tg, B,
+                           blockpref_id);
     return hipGetLastError();
 }
 hipError_t pbwt_persist_prepare(int v, size_t lds, int *occ) {

@@ -48,6 +48,17 @@ def _store(db):
         pass
 
 
+def region_bytes(Mp, K, compact=False):
+    """Bytes a target takes in a context's output pool (cref blocks, count0,
+    forward + backward PBWT rows, each rounded to 256 bytes)."""
+    Mp = np.asarray(Mp, np.int64)
+    K = np.asarray(K, np.int64)
+    up = lambda x: (x + 255) // 256 * 256  # noqa: E731
+    kpad = (K + 63) // 64 * 64
+    rs = 2 * ((K + 31) // 32) if compact else 4 * ((K + 511) // 512) * 8
+    return up((Mp + 15) // 16 * kpad * 2) + up(Mp * 4) + up(2 * Mp * rs * 4)
+
+
 def candidates(Kmax: int):
     """PBWT variants whose capacity holds K rounded up to 64."""
     kpad = (Kmax + 63) // 64 * 64
@@ -88,19 +99,24 @@ def tune(eng: Engine, targets, idx, batches=(128, 256), reserve_bytes: int = 24 
             ms = st.kernel_ms()
             wall = time.perf_counter() - t0
             st.close()
-            done = int((res.Mp > 0).sum())
+            fit = res.Mp > 0
+            done = int(fit.sum())
             dev = sum(v_ for k, v_ in ms.items() if k != "copy")
             row = dict(variant=v["variant"], threads=v["threads"],
                        chunks_per_wave=v["chunks_per_wave"], batch=b, targets=done,
                        ms_per_target=dev / max(done, 1), pbwt_ms=ms["pbwt"], cref_ms=ms["cref"],
-                       wall_s=round(wall, 3))
+                       wall_s=round(wall, 3),
+                       bytes_per_target=float(region_bytes(res.Mp[fit], res.K[fit]).mean())
+                       if done else 0.0)
             table.append(row)
             if log:
                 log(f"tune: {row}")
     best = min(table, key=lambda r: r["ms_per_target"])
     choice = dict(variant=best["variant"], threads=best["threads"],
                   chunks_per_wave=best["chunks_per_wave"], batch=best["batch"],
-                  ms_per_target=best["ms_per_target"], table=table, cached=False)
+                  ms_per_target=best["ms_per_target"],
+                  bytes_per_target=max(r["bytes_per_target"] for r in table),
+                  table=table, cached=False)
     eng.set_pbwt_variant(choice["variant"])
     if use_cache:
         db[key] = {k: v for k, v in choice.items() if k != "cached"}
