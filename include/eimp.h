@@ -9,8 +9,9 @@
  *
  * Threading: an engine handle is used by one host thread at a time, exactly
  * as the reference's GPUEngine is used by one producer thread
- * (src/GPUHandler.cpp:148).  Calls are synchronous: when a call returns,
- * its results are complete in the caller's buffers.
+ * (src/GPUHandler.cpp:148).  Calls are synchronous (when a call returns, its
+ * results are complete in the caller's buffers) except eimp_stream_submit,
+ * whose results eimp_stream_wait collects.
  *
  * Output layout (the reference's drop-in GPU layout, src/GPUHandler.cpp:259,
  * src/GPUEngine.cu:160,182-211): per target 2*Mp rows of 4*Kwords uint32,
