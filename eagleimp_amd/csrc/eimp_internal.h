@@ -121,8 +121,11 @@ struct PanelDev {
     uint32_t N_ad, N_corr, M, pad;
 };
 
-// 32-bit finaliser of MurmurHash3 (Appleby): bijective avalanche mix used for
-// the output digest.  The CPU oracle computes the same sums.
+// Output digest (the CPU oracle computes the same sums): over the PBWT words,
+// sum of rotl32(v ^ g * 0x9E3779B1, g mod 32), g the word's global index;
+// over count0, sum of fmix32(c0 + r * 0x7FEB352D + 0x68E31DA4).  Each term is
+// a bijection of its word.
+// 32-bit finaliser of MurmurHash3 (Appleby).
 __host__ __device__ inline uint32_t fmix32(uint32_t h) {
     h ^= h >> 16;
     h *= 0x85EBCA6Bu;
@@ -131,8 +134,11 @@ __host__ __device__ inline uint32_t fmix32(uint32_t h) {
     h ^= h >> 16;
     return h;
 }
+constexpr uint32_t kDigKey = 0x9E3779B1u;
 __host__ __device__ inline uint32_t dig_word(uint64_t g, uint32_t v) {
-    return fmix32(v + (uint32_t)g * 0x9E3779B1u);
+    const uint32_t x = v ^ ((uint32_t)g * kDigKey);
+    const uint32_t r = (uint32_t)(g & 31);
+    return r ? (x << r) | (x >> (32 - r)) : x;
 }
 __host__ __device__ inline uint32_t dig_count0(uint32_t r, int32_t c0) {
     return fmix32((uint32_t)c0 + r * 0x7FEB352Du + 0x68E31DA4u);

@@ -459,121 +459,170 @@ __global__ __launch_bounds__(SEL_NT) void k_bits_to_list(const TgtDev *__restric
 // a7: condensed reference.  One work item = one 16-row block of a target
 // (segments 8b..8b+7).  Rows 2s (OR of the inconsistencies of segment s,
 // mergeInconsistencies, src/Target.cpp:1040-1059) and 2s+1 (the haplotypes at
-// split s, :986-1003) are first built over all N haplotypes in LDS, G
-// segments per fill, with every thread streaming its words of all the fill's
-// site rows (independent loads).  They are then reduced to the K chosen
-// haplotypes and transposed: one uint16 per haplotype holds the block's 16
-// rows.  Haplotypes K..Kpad-1 get all-ones values (the PBWT kernel's padding).
+// split s, :986-1003) are built in LDS for a tile of up to CREF_TW words of the
+// haplotype axis at a time, every thread streaming two words through all the
+// block's site rows (8 independent loads in flight).  They are then reduced to
+// the K chosen haplotypes of the tile and transposed: one uint16 per haplotype
+// holds the block's 16 rows.  Haplotypes K..Kpad-1 get all-ones values (the
+// PBWT kernel's padding).
 // ---------------------------------------------------------------------------
 constexpr int CREF_NT = 256;
-constexpr uint32_t CREF_LDS = 64 * 1024;  // bytes of staged rows per fill
+constexpr uint32_t CREF_TW = 512;  // words of the haplotype axis per tile (16 rows: 64 KB)
 
-__host__ __device__ inline uint32_t cref_group(uint32_t nw) {
-    const uint32_t g = CREF_LDS / (16u * nw);  // 2 rows of nw words per segment
-    return g < 1 ? 1u : (g > 8 ? 8u : g);
+__host__ __device__ inline uint32_t cref_tile_words(uint32_t nw) {
+    return nw < CREF_TW ? nw : CREF_TW;
 }
 
-// rows of segments [sa, sb) into srow[2*(s-sa)][.] (row 2(s-sa)+1 only if s < S)
-__device__ __forceinline__ void cref_fill(const PanelDev &P, const TgtDev &T, uint32_t sa,
-                                          uint32_t sb, uint32_t nw, uint64_t *srow) {
+// The rows of block b over words [w0, w0 + tw) into srow[row * tw + (w - w0)]
+// (row 2(s-8b)+1 only for s < S).
+__device__ __forceinline__ void cref_fill(const PanelDev &P, const TgtDev &T, uint32_t b,
+                                          uint32_t w0, uint32_t tw, uint32_t nw, uint64_t *srow) {
     const uint64_t *__restrict__ site = P.site + T.s0 * P.site_stride;
     const uint64_t ss = P.site_stride;
     const uint32_t S = T.S, W = T.W;
+    const uint32_t sa = 8 * b, sb = min(8 * b + 8, S + 1);
     const uint64_t lastmask = (P.N_ad & 63) ? ((1ull << (P.N_ad & 63)) - 1) : ~0ull;
-    // sites of the fill: after split sa-1 up to split sb-1 (or the window end)
+    // sites of the block: after split sa-1 up to split sb-1 (or the window end)
     const uint32_t m0 = sa == 0 ? 0u : T.splitsites[sa - 1] + 1;
     const uint32_t m1 = sb <= S ? T.splitsites[sb - 1] + 1 : W;
-    for (uint32_t w = threadIdx.x; w < nw; w += CREF_NT) {
-        const uint64_t wmask = (w == nw - 1) ? lastmask : ~0ull;
+    for (uint32_t wi = threadIdx.x; wi < tw; wi += 2 * CREF_NT) {
+        const uint32_t wj = wi + CREF_NT;  // this thread's second word
+        const bool two = wj < tw;
+        const uint32_t w = w0 + wi, w2 = w0 + (two ? wj : wi);
+        const uint64_t mask0 = (w == nw - 1) ? lastmask : ~0ull;
+        const uint64_t mask1 = (w2 == nw - 1) ? lastmask : ~0ull;
         uint32_t s = sa;
-        uint32_t next = s < S ? T.splitsites[s] : 0xFFFFFFFFu;
-        uint64_t a = 0;
-        // one pass over the sites, loads issued 4 at a time
+        uint64_t a0 = 0, a1 = 0;
+        // one pass over the sites; per group of 4 sites the site-row words, the
+        // genotype bits and the split bits are independent loads
         for (uint32_t mb = m0; mb < m1; mb += 4) {
-            uint64_t v[4];
+            uint64_t v0[4], v1[4];
+            uint32_t sp[4], is0[4], is2[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const uint32_t m = mb + u;
-                v[u] = m < m1 ? site[(uint64_t)m * ss + w] : 0ull;
+                const bool in = m < m1;
+                const uint32_t mw = in ? m : m0;
+                v0[u] = in ? site[(uint64_t)m * ss + w] : 0ull;
+                v1[u] = in ? site[(uint64_t)m * ss + w2] : 0ull;
+                sp[u] = (uint32_t)(T.splits[mw >> 6] >> (mw & 63)) & 1u;
+                is0[u] = (uint32_t)(T.gt[2 * (mw >> 6)] >> (mw & 63)) & 1u;
+                is2[u] = (uint32_t)(T.gt[2 * (mw >> 6) + 1] >> (mw & 63)) & 1u;
             }
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const uint32_t m = mb + u;
                 if (m >= m1) break;
-                if (m == next) {  // split: close segment s, its haplotypes are row 2s+1
-                    srow[(uint64_t)(2 * (s - sa)) * nw + w] = a & wmask;
-                    srow[(uint64_t)(2 * (s - sa) + 1) * nw + w] = v[u];
-                    a = 0;
+                if (sp[u]) {  // split: close segment s, its haplotypes are row 2s+1
+                    const uint32_t r0 = 2 * (s - sa);
+                    srow[(uint64_t)r0 * tw + wi] = a0 & mask0;
+                    srow[(uint64_t)(r0 + 1) * tw + wi] = v0[u];
+                    if (two) {
+                        srow[(uint64_t)r0 * tw + wj] = a1 & mask1;
+                        srow[(uint64_t)(r0 + 1) * tw + wj] = v1[u];
+                    }
+                    a0 = a1 = 0;
                     s++;
-                    next = s < S ? T.splitsites[s] : 0xFFFFFFFFu;
                 } else {
-                    const uint64_t g0 = T.gt[2 * (m >> 6)], g2 = T.gt[2 * (m >> 6) + 1];
-                    const uint64_t is0 = (g0 >> (m & 63)) & 1ull, is2 = (g2 >> (m & 63)) & 1ull;
                     // HomRef: the ones are inconsistent; HomAlt: the zeros; het/missing: none
-                    const uint64_t flip = is2 ? ~0ull : 0ull;
-                    const uint64_t keep = (is0 ^ is2) ? ~0ull : 0ull;
-                    a |= (v[u] ^ flip) & keep;
+                    const uint64_t flip = is2[u] ? ~0ull : 0ull;
+                    const uint64_t keep = (is0[u] ^ is2[u]) ? ~0ull : 0ull;
+                    a0 |= (v0[u] ^ flip) & keep;
+                    a1 |= (v1[u] ^ flip) & keep;
                 }
             }
         }
-        if (sb == S + 1) srow[(uint64_t)(2 * (S - sa)) * nw + w] = a & wmask;  // last segment
+        if (sb == S + 1) {  // the last segment has no split row
+            const uint32_t r0 = 2 * (S - sa);
+            srow[(uint64_t)r0 * tw + wi] = a0 & mask0;
+            if (two) srow[(uint64_t)r0 * tw + wj] = a1 & mask1;
+        }
     }
 }
 
+// first k in [lo, hi) with bh[k] >= x (bh ascending)
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *bh, uint32_t lo, uint32_t hi,
+                                                    uint32_t x) {
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (bh[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
 // Block b of a target, K chosen haplotypes by index (any target).
-// Haplotype-outer: thread t takes haplotypes k = t, t + CREF_NT, ... and reads
-// the staged rows at bit bh[k] (consecutive lanes read nearby words: bh is
-// ascending), so one register accumulates a haplotype's 16 row bits.  A block
-// whose segments take several fills keeps the partial values in vacc (LDS).
-// count0 from popcounts of each staged row under the best-hap bitmap.
+// Haplotype-outer: in each tile thread t takes the chosen haplotypes k = klo+t,
+// klo+t+CREF_NT, ... of the tile and reads the staged rows at bit bh[k]
+// (consecutive lanes read nearby words: bh is ascending), so one register
+// holds a haplotype's 16 row bits.  count0 from popcounts of the staged rows
+// under the best-hap bitmap.
 __device__ __forceinline__ void cref_block_gather(const PanelDev &P, const TgtDev &T, uint32_t b,
-                                                  uint64_t *srow, uint32_t *cnt, uint16_t *vacc) {
+                                                  uint64_t *srow, uint32_t *cnt) {
     const uint32_t tid = threadIdx.x, l = lane_id();
     const uint32_t K = T.K, S = T.S, Kpad = T.Kpad, Mp = T.Mp;
     const uint32_t nw = (P.N_ad + 63) / 64;
-    const uint32_t nw32 = 2 * nw;
-    const uint32_t G = cref_group(nw);
+    const uint32_t tw = cref_tile_words(nw);
     const bool ident = (T.flags & TF_IDENTITY) != 0;
     const uint64_t *__restrict__ bhf = T.bhflags;
+    const uint32_t *__restrict__ bh = T.bh;
+    const uint32_t sb = min(8 * b + 8, S + 1);
+    const uint32_t nrows = 2 * (sb - 8 * b) - (sb == S + 1 ? 1u : 0u);
     if (tid < 16) cnt[tid] = 0;
     uint16_t *__restrict__ dst = T.cref + (uint64_t)b * Kpad;
-    const uint32_t send = min(8 * b + 8, S + 1);
-    for (uint32_t sa = 8 * b; sa < send; sa += G) {
-        const uint32_t sb = min(sa + G, send);
-        cref_fill(P, T, sa, sb, nw, srow);
+    for (uint32_t k = K + tid; k < Kpad; k += CREF_NT) dst[k] = 0xFFFFu;
+    uint32_t ones[16];  // this thread's share of the ones among the chosen haplotypes, per row
+#pragma unroll
+    for (int rr = 0; rr < 16; rr++) ones[rr] = 0;
+    uint32_t klo = 0;  // first chosen haplotype of the tile
+    for (uint32_t w0 = 0; w0 < nw; w0 += tw) {
+        const uint32_t twc = min(tw, nw - w0);
+        cref_fill(P, T, b, w0, twc, nw, srow);
         __syncthreads();
-        const uint32_t nrows = 2 * (sb - sa) - (sb == S + 1 ? 1u : 0u);
-        const uint32_t rb0 = 2 * (sa - 8 * b);
-        // ones among the chosen haplotypes, per row
-        for (uint32_t rr = 0; rr < nrows; rr++) {
-            const uint64_t *row = srow + (uint64_t)rr * nw;
-            uint32_t ones = 0;
-            for (uint32_t w = tid; w < nw; w += CREF_NT) ones += (uint32_t)__popcll(row[w] & bhf[w]);
-            ones = wave_sum(ones);
-            if (l == 0 && ones) atomicAdd(&cnt[rb0 + rr], ones);
+        for (uint32_t wi = tid; wi < twc; wi += CREF_NT) {
+            const uint64_t f = bhf[w0 + wi];
+#pragma unroll
+            for (uint32_t rr = 0; rr < 16; rr++)
+                if (rr < nrows) ones[rr] += (uint32_t)__popcll(srow[(uint64_t)rr * twc + wi] & f);
         }
-        const bool first = sa == 8 * b, last = sb == send;
+        const uint32_t idx_end = 64 * (w0 + twc);
+        const uint32_t khi = w0 + twc >= nw ? K : (ident ? min(K, idx_end) : lower_bound_u32(bh, klo, K, idx_end));
         const uint32_t *rows32 = (const uint32_t *)srow;
-        for (uint32_t k = tid; k < Kpad; k += CREF_NT) {
-            uint32_t v = 0xFFFFu;  // haplotypes K..Kpad-1: the PBWT kernel's padding
-            if (k < K) {
-                const uint32_t idx = ident ? k : T.bh[k];
-                const uint32_t *col = rows32 + (idx >> 5);
-                const uint32_t sh = idx & 31;
-                v = first ? 0u : (uint32_t)vacc[k];
+        const uint32_t st32 = 2 * twc;  // row stride in uint32
+        // four haplotypes per thread per round: their indices are loaded together
+        for (uint32_t kb = klo + tid; kb < khi; kb += 4 * CREF_NT) {
+            uint32_t idx[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t k = kb + u * CREF_NT;
+                idx[u] = k < khi ? (ident ? k : bh[k]) - 64 * w0 : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t k = kb + u * CREF_NT;
+                if (k >= khi) break;
+                const uint32_t *col = rows32 + (idx[u] >> 5);
+                const uint32_t sh = idx[u] & 31;
+                uint32_t v = 0;
                 if (nrows == 16) {
 #pragma unroll
-                    for (uint32_t rr = 0; rr < 16; rr++) v |= ((col[rr * nw32] >> sh) & 1u) << rr;
+                    for (uint32_t rr = 0; rr < 16; rr++) v |= ((col[rr * st32] >> sh) & 1u) << rr;
                 } else {
-                    for (uint32_t rr = 0; rr < nrows; rr++)
-                        v |= ((col[rr * nw32] >> sh) & 1u) << (rb0 + rr);
+                    for (uint32_t rr = 0; rr < nrows; rr++) v |= ((col[rr * st32] >> sh) & 1u) << rr;
                 }
+                dst[k] = (uint16_t)v;
             }
-            if (last) dst[k] = (uint16_t)v;
-            else vacc[k] = (uint16_t)v;
         }
+        klo = khi;
         __syncthreads();
     }
+#pragma unroll
+    for (uint32_t rr = 0; rr < 16; rr++) {
+        if (rr < nrows) {
+            const uint32_t o = wave_sum(ones[rr]);
+            if (l == 0 && o) atomicAdd(&cnt[rr], o);
+        }
+    }
+    __syncthreads();
     if (tid < 16) {
         const uint32_t r = 16 * b + tid;
         if (r < Mp) T.count0[r] = (int32_t)K - (int32_t)cnt[tid];
@@ -590,22 +639,20 @@ __device__ __forceinline__ void cref_block_ident(const PanelDev &P, const TgtDev
                                                  uint64_t *srow, uint32_t *cnt) {
     const uint32_t tid = threadIdx.x, l = lane_id();
     const uint32_t K = T.K, S = T.S, Kpad = T.Kpad, Mp = T.Mp;
-    const uint32_t nw = (P.N_ad + 63) / 64;  // == Kpad / 64 (K == N_ad)
-    const uint32_t G = cref_group(nw);
+    const uint32_t nw = (P.N_ad + 63) / 64;  // == Kpad / 64 (K == N_ad), at most CREF_TW
     uint32_t V[GPT][32];  // two uint16 values per register
 #pragma unroll
     for (int e = 0; e < GPT; e++)
 #pragma unroll
         for (int i = 0; i < 32; i++) V[e][i] = 0;
     if (tid < 16) cnt[tid] = 0;
-    const uint32_t send = min(8 * b + 8, S + 1);
-    for (uint32_t sa = 8 * b; sa < send; sa += G) {
-        const uint32_t sb = min(sa + G, send);
-        cref_fill(P, T, sa, sb, nw, srow);
+    const uint32_t sb = min(8 * b + 8, S + 1);
+    {
+        cref_fill(P, T, b, 0, nw, nw, srow);
         __syncthreads();
-        const uint32_t nrows = 2 * (sb - sa) - (sb == S + 1 ? 1u : 0u);
+        const uint32_t nrows = 2 * (sb - 8 * b) - (sb == S + 1 ? 1u : 0u);
         for (uint32_t rr = 0; rr < nrows; rr++) {
-            const uint32_t rb = 2 * (sa - 8 * b) + rr;
+            const uint32_t rb = rr;
             const uint64_t *row = srow + (uint64_t)rr * nw;
             uint32_t ones = 0;
 #pragma unroll
@@ -657,8 +704,7 @@ __global__ __launch_bounds__(CREF_NT) void k_cref(PanelDev P, const TgtDev *__re
     extern __shared__ uint64_t srow[];
     __shared__ uint32_t cnt[16];
     const CrefItem it = items[blockIdx.x];
-    uint16_t *vacc = (uint16_t *)(srow + 2 * cref_group((P.N_ad + 63) / 64) * ((P.N_ad + 63) / 64));
-    for (uint32_t b = it.b0; b < it.b1; b++) cref_block_gather(P, tg[it.t], b, srow, cnt, vacc);
+    for (uint32_t b = it.b0; b < it.b1; b++) cref_block_gather(P, tg[it.t], b, srow, cnt);
 }
 
 template <int GPT>
@@ -771,6 +817,12 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
     const uint32_t och = ch0 + ln;         // chunk stored by this lane
     const bool ostore = ln < (uint32_t)NS && och < nch;
     const uint32_t ln4 = 4u * ln;
+    // digest of this lane's words: g = gbase + step*KW + 4*och (+0..3); the
+    // key g * kDigKey and the rotation -g (alignbit uses 5 bits) advance by
+    // a constant per step
+    const uint32_t g0 = (uint32_t)gbase + 4u * och;
+    uint32_t dkey = g0 * kDigKey, dneg = 0u - g0;
+    const uint32_t dkey_step = KW * kDigKey;
 
     for (uint32_t step = 0; step < Mp; ++step) {
         const uint32_t r = rev ? Mp - 1 - step : step;
@@ -842,11 +894,19 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
             if (second) dst[1] = make_uint2(ohi, o3);
             else if (4 * och + 2 < RS) dst[1] = make_uint2(0u, 0u);
             if (dig) {
-                const uint64_t g = gbase + (uint64_t)step * KW + 4ull * och;
-                dsum += (uint64_t)dig_word(g, olo) + (uint64_t)dig_word(g + 1, o1);
-                if (second) dsum += (uint64_t)dig_word(g + 2, ohi) + (uint64_t)dig_word(g + 3, o3);
+                // rotl(v ^ key, g mod 32) == rotr by -g: one v_alignbit each
+                dsum += __builtin_amdgcn_alignbit(olo ^ dkey, olo ^ dkey, dneg);
+                const uint32_t x1 = o1 ^ (dkey + kDigKey);
+                dsum += __builtin_amdgcn_alignbit(x1, x1, dneg - 1u);
+                if (second) {
+                    const uint32_t x2 = ohi ^ (dkey + 2u * kDigKey), x3 = o3 ^ (dkey + 3u * kDigKey);
+                    dsum += __builtin_amdgcn_alignbit(x2, x2, dneg - 2u);
+                    dsum += __builtin_amdgcn_alignbit(x3, x3, dneg - 3u);
+                }
             }
         }
+        dkey += dkey_step;
+        dneg -= KW;
         // zero slots after the logical row
         const uint32_t tail = nch + threadIdx.x;
         if (tail < nslots) ((uint4 *)row)[tail] = make_uint4(0u, 0u, 0u, 0u);
@@ -1052,13 +1112,11 @@ __global__ __launch_bounds__(CREF_NT) void k_cref_persist(PanelDev P, const TgtD
     extern __shared__ uint64_t srow[];
     __shared__ uint32_t cnt[16];
     const uint32_t nq = B->ncref, n = B->n;
-    const uint32_t nw = (P.N_ad + 63) / 64;
-    uint16_t *vacc = (uint16_t *)(srow + 2 * cref_group(nw) * nw);
     for (;;) {
         uint32_t i, t;
         cref_pop(&B->cref_ctr, nq, n, blockpref, i, t);
         if (i >= nq) break;
-        cref_block_gather(P, tg[t], i - blockpref[t], srow, cnt, vacc);
+        cref_block_gather(P, tg[t], i - blockpref[t], srow, cnt);
     }
 }
 
@@ -1184,14 +1242,10 @@ hipError_t launch_bits_to_list(const TgtDev *tg, uint32_t n, uint32_t nbits_bh, 
     return hipGetLastError();
 }
 
-// dynamic LDS of the cref kernels: the staged rows of one fill, plus the
-// partial values of a block that takes several fills (K up to maxK)
+// dynamic LDS of the cref kernels: the 16 staged rows of one tile
 size_t cref_lds_bytes(const PanelDev &P, uint32_t maxK) {
-    const uint32_t nw = (P.N_ad + 63) / 64;
-    const uint32_t G = cref_group(nw);
-    size_t lds = (size_t)2 * G * nw * sizeof(uint64_t);
-    if (G < 8) lds += ((size_t)maxK + 63) / 64 * 64 * sizeof(uint16_t);
-    return lds;
+    (void)maxK;
+    return (size_t)16 * cref_tile_words((P.N_ad + 63) / 64) * sizeof(uint64_t);
 }
 
 hipError_t launch_cref(const PanelDev &P, const TgtDev *tg, const CrefItem *items, uint32_t nitems,

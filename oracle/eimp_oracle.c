@@ -429,10 +429,16 @@ void eo_map_interval(const uint32_t *pbwt, uint64_t K, uint64_t Mp, const int32_
 
 /* ---------------------------------------------------------------------- */
 /* Digest of the logical output (both sides compute the same sums): the sum  */
-/* over every logical word of fmix32(v + g * 0x9E3779B1), g the word's index */
-/* in (direction, stored row, word) order, and over count0 likewise.         */
-/* fmix32 is bijective, so any single differing word changes the sum.       */
+/* over every logical word v of rotl32(v ^ g * 0x9E3779B1, g mod 32), g the  */
+/* word's index in (direction, stored row, word) order, and over count0 of   */
+/* fmix32(c0 + r * 0x7FEB352D + 0x68E31DA4).  Each term is a bijection of    */
+/* its word, so any single differing word changes the sum.                  */
 /* ---------------------------------------------------------------------- */
+static inline uint32_t dig_term(uint64_t g, uint32_t v) {
+    const uint32_t x = v ^ ((uint32_t)g * 0x9E3779B1u);
+    const uint32_t r = (uint32_t)(g & 31);
+    return r ? (x << r) | (x >> (32 - r)) : x;
+}
 static inline uint32_t fmix32(uint32_t h) {
     /* MurmurHash3 32-bit finaliser (Appleby) */
     h ^= h >> 16;
@@ -458,7 +464,7 @@ void eo_digest(const uint32_t *pbwt, uint64_t row_stride, uint64_t K, uint64_t M
     for (uint64_t q = 0; q < 2 * Mp; q++)
         for (uint64_t w = 0; w < KW; w++) {
             uint64_t g = q * KW + w;
-            s += fmix32(pbwt[q * row_stride + w] + (uint32_t)g * 0x9E3779B1u);
+            s += dig_term(g, pbwt[q * row_stride + w]);
         }
     uint64_t c = 0;
     for (uint64_t r = 0; r < Mp; r++)

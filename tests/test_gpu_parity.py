@@ -175,3 +175,24 @@ def test_pbwt_variants_agree(built, small, variant):
             assert int(res.dig_pbwt[t]) == refs[t]["dig_pbwt"], f"variant {v} target {t}"
             assert int(res.dig_count0[t]) == refs[t]["dig_count0"]
     eng.close()
+
+
+def test_wide_panel_cref_tiles(built):
+    """N > 32,768 haplotypes: the cref kernels stage the haplotype axis in
+    several tiles per block (subset targets only: K is at most 24,576)."""
+    panel = synth.make_panel(17, N=33500, M=700, F=40, T=6)
+    pv = oracle.PanelView(panel.hap, panel.site, panel.cM)
+    eng = _panel_engine(panel)
+    tg = synth.make_targets(panel, 17, 4, kset=[40, 3000, 20000, 24000], wmin=300, wmax=700,
+                            miss=0.0, miss_hi=0.03)
+    _run_and_compare(panel, pv, eng, tg.gt, tg.s0, tg.W, tg.K, tag="wide")
+    st = eng.stream(4, 700, 24000, 1 << 30)
+    st.submit(tg.gt, tg.s0, tg.W, tg.K)
+    res, nd, _ = st.wait()
+    assert nd == 0
+    for t in range(4):
+        r = oracle.process_target(pv, tg.gt[t], tg.s0[t], tg.W[t], tg.K[t], keep=False)
+        assert (int(res.Mp[t]), int(res.dig_pbwt[t]), int(res.dig_count0[t])) == \
+            (r["Mp"], r["dig_pbwt"], r["dig_count0"]), f"stream wide target {t}"
+    st.close()
+    eng.close()

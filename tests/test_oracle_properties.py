@@ -266,3 +266,34 @@ def test_digest_detects_single_word_change(data):
     pb[[0, 1]] = pb[[1, 0]]  # swapped rows
     if not np.array_equal(pb, r["pbwt"]):
         assert oracle.digest(pb, K, r["count0"])[0] != base[0]
+
+
+def test_digest_formula():
+    """oracle.digest against a numpy evaluation of its definition: sum over
+    logical words of rotl32(v ^ g*0x9E3779B1, g mod 32), and over count0 of
+    fmix32(c0 + r*0x7FEB352D + 0x68E31DA4)."""
+    rng = np.random.default_rng(7)
+    K, Mp = 77, 9
+    KW = oracle.pbwt_row_words(K)
+    rows = rng.integers(0, 2**32, size=(2 * Mp, KW + 3), dtype=np.uint64).astype(np.uint32)
+    c0 = rng.integers(0, K + 1, size=Mp).astype(np.int32)
+    g = np.arange(2 * Mp * KW, dtype=np.uint64)
+    v = rows[:, :KW].reshape(-1).astype(np.uint64)
+    x = (v ^ ((g * 0x9E3779B1) & 0xFFFFFFFF)) & 0xFFFFFFFF
+    r = g & 31
+    t = ((x << r) | (x >> ((32 - r) & 63))) & 0xFFFFFFFF
+    t = np.where(r == 0, x, t)
+
+    def fmix32(h):
+        h = h & 0xFFFFFFFF
+        h ^= h >> 16
+        h = (h * 0x85EBCA6B) & 0xFFFFFFFF
+        h ^= h >> 13
+        h = (h * 0xC2B2AE35) & 0xFFFFFFFF
+        h ^= h >> 16
+        return h
+
+    rr = np.arange(Mp, dtype=np.uint64)
+    c = fmix32((c0.astype(np.uint64) & 0xFFFFFFFF) + ((rr * 0x7FEB352D) & 0xFFFFFFFF) + 0x68E31DA4)
+    want = (int(t.sum()) % 2**64, int(c.sum()) % 2**64)
+    assert oracle.digest(rows, K, c0) == want
