@@ -120,34 +120,116 @@ __global__ __launch_bounds__(256) void k_panel_mono(PanelDev P, uint64_t *__rest
 }
 
 // ---------------------------------------------------------------------------
-// a3: distances.  grid (ceil(N_ad/16), nsel), block 256, a wave per haplotype
+// a3: distances for groups of DIST_G targets.  grid (ceil(N_ad/16),
+// ceil(n/DIST_G)), block 256: 16 haplotypes (4 per wave) x DIST_G targets per
+// workgroup.  Over the union of the group's windows, 64 panel words at a time,
+// the group's genotype words are staged in LDS shifted to panel alignment
+// (sites outside a target's window zero); each lane then reads one word of its
+// 4 haplotype rows once for all targets of the group, so a panel word is read
+// once per group rather than once per target.  nerr = sum of
+// popcount((is0 & h) | (is2 & ~h)) over the window (compare_withPreInit,
+// src/Datatypes.cpp:107-116).
 // ---------------------------------------------------------------------------
+constexpr int DIST_G = 8;
+
+// genotype words (is0, is2) of T for panel word p, bits of sites outside
+// [s0, s0 + W) zero
+__device__ __forceinline__ void gt_panel_word(const TgtDev &T, uint64_t p, uint64_t &a,
+                                              uint64_t &b) {
+    const uint64_t nq = ((uint64_t)T.W + 63) / 64;  // window words with data
+    const int64_t d = (int64_t)(64 * p) - (int64_t)T.s0;  // window bit of panel bit 0
+    auto at = [&](int64_t q, int k) -> uint64_t {
+        return (q >= 0 && (uint64_t)q < nq) ? T.gt[2 * q + k] : 0ull;
+    };
+    if (d >= 0) {
+        const int64_t q = d >> 6;
+        const uint32_t sh = (uint32_t)(d & 63);
+        a = at(q, 0);
+        b = at(q, 1);
+        if (sh) {
+            a = (a >> sh) | (at(q + 1, 0) << (64 - sh));
+            b = (b >> sh) | (at(q + 1, 1) << (64 - sh));
+        }
+    } else if (d > -64) {
+        const uint32_t sh = (uint32_t)(-d);
+        a = at(0, 0) << sh;
+        b = at(0, 1) << sh;
+    } else {
+        a = b = 0ull;
+    }
+    // panel bits at or beyond the window end
+    const int64_t end = (int64_t)T.s0 + (int64_t)T.W - (int64_t)(64 * p);
+    const uint64_t keep = end >= 64 ? ~0ull : (end <= 0 ? 0ull : ((1ull << end) - 1));
+    a &= keep;
+    b &= keep;
+}
+
 __global__ __launch_bounds__(256) void k_distance(PanelDev P, const TgtDev *__restrict__ tg,
-                                                 const uint32_t *__restrict__ tlist,
+                                                 const uint32_t *__restrict__ tlist, uint32_t nlist,
                                                  const uint32_t *__restrict__ d_n) {
-    // tlist: targets that select; without it every target of the batch (size *d_n)
-    if (!tlist && blockIdx.y >= *d_n) return;
-    const TgtDev &T = tg[tlist ? tlist[blockIdx.y] : blockIdx.y];
-    if (!(T.flags & TF_SELECT)) return;
-    const uint32_t wave = threadIdx.x >> 6, l = lane_id();
-    const uint64_t Ww = (T.W + 63) / 64;
-    const uint64_t *__restrict__ gt = T.gt;
-#pragma unroll 1
-    for (uint32_t hh = 0; hh < 4; hh++) {
-        const uint32_t h = blockIdx.x * 16 + wave * 4 + hh;
-        if (h >= P.N_ad) break;
-        if (excluded_hap(P, T.ignore, h)) {
-            if (l == 0) T.nerr[h] = kExcluded;
-            continue;
+    // tlist: targets that select (nlist of them); without it every target of the
+    // batch (size *d_n), those that do not select are skipped
+    __shared__ uint64_t sg[DIST_G][64][2];
+    const uint32_t n = tlist ? nlist : *d_n;
+    const uint32_t j0 = blockIdx.y * DIST_G;
+    if (j0 >= n) return;
+    const TgtDev *Tj[DIST_G];
+    bool act[DIST_G];
+    uint64_t plo = ~0ull, phi = 0;
+#pragma unroll
+    for (int j = 0; j < DIST_G; j++) {
+        const uint32_t q = j0 + j;
+        act[j] = q < n;
+        Tj[j] = act[j] ? &tg[tlist ? tlist[q] : q] : tg;
+        act[j] = act[j] && (Tj[j]->flags & TF_SELECT);
+        if (act[j]) {
+            plo = min(plo, Tj[j]->s0 / 64);
+            phi = max(phi, (Tj[j]->s0 + Tj[j]->W + 63) / 64);
         }
-        const uint64_t *__restrict__ row = P.hap + (uint64_t)h * P.hap_stride;
-        uint32_t e = 0;
-        for (uint64_t i = l; i < Ww; i += 64) {
-            uint64_t hv = funnel(row, T.s0 + 64 * i, P.hap_stride);
-            e += __popcll((gt[2 * i] & hv) | (gt[2 * i + 1] & ~hv));
+    }
+    if (phi == 0) return;
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, l = lane_id();
+    const uint32_t hb = blockIdx.x * 16 + wave * 4;
+    uint32_t e[4][DIST_G];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < DIST_G; j++) e[i][j] = 0;
+    for (uint64_t c0 = plo; c0 < phi; c0 += 64) {
+        for (uint32_t q = tid; q < DIST_G * 64; q += 256) {
+            const uint32_t j = q / 64, pi = q % 64;
+            uint64_t a = 0, b = 0;
+#pragma unroll
+            for (int jj = 0; jj < DIST_G; jj++)  // static indexing of Tj / act
+                if (jj == (int)j && act[jj] && c0 + pi < phi) gt_panel_word(*Tj[jj], c0 + pi, a, b);
+            sg[j][pi][0] = a;
+            sg[j][pi][1] = b;
         }
-        e = wave_sum(e);
-        if (l == 0) T.nerr[h] = e;
+        __syncthreads();
+        const uint64_t p = c0 + l;
+        uint64_t hv[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            hv[i] = (hb + i < P.N_ad && p < phi) ? P.hap[(uint64_t)(hb + i) * P.hap_stride + p] : 0ull;
+#pragma unroll
+        for (int j = 0; j < DIST_G; j++) {
+            if (!act[j]) continue;
+            const uint64_t g0 = sg[j][l][0], g2 = sg[j][l][1];
+#pragma unroll
+            for (int i = 0; i < 4; i++) e[i][j] += (uint32_t)__popcll((g0 & hv[i]) | (g2 & ~hv[i]));
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < DIST_G; j++) {
+        if (!act[j]) continue;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const uint32_t h = hb + i;
+            const uint32_t s = wave_sum(e[i][j]);
+            if (l == 0 && h < P.N_ad)
+                Tj[j]->nerr[h] = excluded_hap(P, Tj[j]->ignore, h) ? kExcluded : s;
+        }
     }
 }
 
@@ -1213,8 +1295,8 @@ hipError_t launch_panel_mono(const PanelDev &P, uint64_t *pand, uint64_t *por, h
 hipError_t launch_distance(const PanelDev &P, const TgtDev *tg, const uint32_t *tlist,
                            uint32_t nsel, hipStream_t s) {
     if (!nsel) return hipSuccess;
-    hipLaunchKernelGGL(k_distance, dim3((P.N_ad + 15) / 16, nsel), dim3(256), 0, s, P, tg, tlist,
-                       (const uint32_t *)nullptr);
+    hipLaunchKernelGGL(k_distance, dim3((P.N_ad + 15) / 16, (nsel + DIST_G - 1) / DIST_G), dim3(256),
+                       0, s, P, tg, tlist, nsel, (const uint32_t *)nullptr);
     return hipGetLastError();
 }
 hipError_t launch_select(const PanelDev &P, const TgtDev *tg, uint32_t n, hipStream_t s) {
@@ -1332,8 +1414,8 @@ hipError_t launch_prep_dev(int cls, const PanelDev &P, const TgtDev *tg, const u
                            uint32_t nmax, uint32_t max_ww, hipStream_t s) {
     switch (cls) {
     case 0:
-        hipLaunchKernelGGL(k_distance, dim3((P.N_ad + 15) / 16, nmax), dim3(256), 0, s, P, tg,
-                           (const uint32_t *)nullptr, d_n);
+        hipLaunchKernelGGL(k_distance, dim3((P.N_ad + 15) / 16, (nmax + DIST_G - 1) / DIST_G),
+                           dim3(256), 0, s, P, tg, (const uint32_t *)nullptr, 0u, d_n);
         break;
     case 1:
         hipLaunchKernelGGL(k_select, dim3(nmax), dim3(SEL_NT), 0, s, P, tg, d_n);
