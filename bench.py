@@ -92,7 +92,8 @@ def serve_bench(args):
     eng = Engine(0)
     eng.initialize(2, panel.M, panel.N, panel.N, panel.site)
     eng.set_panel_extras(panel.cM, hap=panel.hap)
-    srv = serve.Server(eng, pool, int(pool.W.max()), int(pool.K.max()), contexts=args.contexts + 1)
+    srv = serve.Server(eng, pool, int(pool.W.max()), int(pool.K.max()), contexts=args.contexts,
+                       max_batch=args.serve_batch)
     warm = serve.make_requests([0.0], [64], [0], len(pool.W))
     serve.run_requests(srv, warm, realtime=False)
     runs = []
@@ -121,7 +122,8 @@ def serve_bench(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
         "data": "synthetic",
         "config": {"workload": cfg["name"], "N": panel.N, "M": panel.M,
-                   "pool_targets": len(pool.W), "contexts": args.contexts + 1,
+                   "pool_targets": len(pool.W), "contexts": args.contexts,
+                   "max_targets_per_submission": args.serve_batch,
                    "rates": list(args.rates), "requests_per_rate": args.requests},
         "p50_ms": worst["p50_ms"], "p99_ms": worst["p99_ms"],
         "serving": runs, "submissions": subs, "deferred": deferred,
@@ -139,6 +141,8 @@ def main():
     ap.add_argument("--rates", type=float, nargs="+", default=[20.0, 200.0],
                     help="workload 5: request arrival rates (batches/s)")
     ap.add_argument("--requests", type=int, default=120, help="workload 5: requests per rate")
+    ap.add_argument("--serve-batch", type=int, default=512,
+                    help="workload 5: most targets per context submission")
     ap.add_argument("--targets", type=int, default=None, help="targets in the workload")
     ap.add_argument("--batch", type=int, default=None, help="targets per step")
     ap.add_argument("--variant", type=int, default=-1, help="PBWT kernel variant (-1 auto)")

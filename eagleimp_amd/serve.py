@@ -160,7 +160,8 @@ def main(argv=None):
     ap.add_argument("--rate", type=float, nargs="+", default=[20.0, 200.0],
                     help="arrival rates in request batches per second")
     ap.add_argument("--requests", type=int, default=100, help="requests per rate")
-    ap.add_argument("--contexts", type=int, default=3)
+    ap.add_argument("--contexts", type=int, default=2)
+    ap.add_argument("--max-batch", type=int, default=512, help="most targets per submission")
     ap.add_argument("--pool", type=int, default=None, help="targets in the pool (default 1024)")
     ap.add_argument("--seed", type=int, default=5)
     args = ap.parse_args(argv)
@@ -169,7 +170,8 @@ def main(argv=None):
     eng = Engine(0)
     eng.initialize(2, panel.M, panel.N, panel.N, panel.site)
     eng.set_panel_extras(panel.cM, hap=panel.hap)
-    srv = Server(eng, pool, int(pool.W.max()), int(pool.K.max()), contexts=args.contexts)
+    srv = Server(eng, pool, int(pool.W.max()), int(pool.K.max()), contexts=args.contexts,
+                 max_batch=args.max_batch)
     out = {"workload": c["name"], "contexts": args.contexts, "runs": []}
     # warm-up pass (allocations, first launches)
     warm = make_requests([0.0], [min(64, len(pool.W))], [0], len(pool.W))

@@ -111,7 +111,12 @@ def tune(eng: Engine, targets, idx, batches=(128, 256), reserve_bytes: int = 24 
             table.append(row)
             if log:
                 log(f"tune: {row}")
-    best = min(table, key=lambda r: r["ms_per_target"])
+    # near-ties (within 2%) go to the lowest variant index, so that repeated
+    # runs pick the same kernel; then the fastest submission size of it
+    fastest = min(r["ms_per_target"] for r in table)
+    near = [r for r in table if r["ms_per_target"] <= 1.02 * fastest]
+    v0 = min(r["variant"] for r in near)
+    best = min((r for r in near if r["variant"] == v0), key=lambda r: r["ms_per_target"])
     choice = dict(variant=best["variant"], threads=best["threads"],
                   chunks_per_wave=best["chunks_per_wave"], batch=best["batch"],
                   ms_per_target=best["ms_per_target"],

@@ -29,7 +29,7 @@ class _Stream:
     def kernel_ms(self):
         v = self.eng.variant
         # variant 3 is fastest per target, and larger batches amortise better
-        per = {2: 2.0, 3: 1.0, 4: 1.5}.get(v, 9.0)
+        per = self.eng.per.get(v, 9.0)
         return {"distance": 0.0, "select": 0.1, "monos": 0.0, "splits": 0.1,
                 "cref": 0.2 * self.n, "pbwt": per * self.n * (1.0 if self.batch > 128 else 1.2),
                 "copy": 5.0}
@@ -42,9 +42,11 @@ class _Engine:
     Nad = Ncorr = 20000
     M = 200000
 
-    def __init__(self):
+    def __init__(self, per=None):
         self.variant = -1
         self.submits = 0
+        # device ms per target of each PBWT variant
+        self.per = per or {2: 2.0, 3: 1.0, 4: 1.5}
 
     def set_pbwt_variant(self, v):
         self.variant = v
@@ -75,3 +77,10 @@ def test_tune_picks_fastest_and_caches(tmp_path, monkeypatch):
     eng2 = _Engine()
     again = tune.tune(eng2, tg, range(300), batches=(128, 256))
     assert again["cached"] and again["variant"] == 3 and eng2.submits == 0 and eng2.variant == 3
+
+
+def test_near_tie_goes_to_lowest_variant(tmp_path, monkeypatch):
+    monkeypatch.setattr(tune, "CACHE", str(tmp_path / "tune.json"))
+    eng = _Engine(per={2: 1.005, 3: 1.0, 4: 1.5})  # 2 is within 2% of 3
+    best = tune.tune(eng, _Targets(300, 10000), range(300), batches=(128, 256), use_cache=False)
+    assert (best["variant"], best["batch"]) == (2, 256)
