@@ -70,6 +70,11 @@ struct BatchDev {
     uint32_t pbwt_ctr;
     uint32_t ncref_id;    // cref blocks of whole-panel (identity) targets
     uint32_t cref_id_ctr;
+    // cref queue order per kind ([0] subset, [1] identity): 0 target-major
+    // (prefix sums in blockpref), else block-major over the nrr targets listed
+    // in blockpref (item i = block i / nrr of target i % nrr)
+    uint32_t rr[2];
+    uint32_t nrr[2];
 };
 
 // cref work item: target t, blocks [b0, b1)

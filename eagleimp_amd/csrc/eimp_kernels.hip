@@ -1065,6 +1065,23 @@ __device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t *s_w,
     return before + inc - v;
 }
 
+// block-wide maximum (PLAN_NT threads)
+__device__ __forceinline__ uint64_t block_max64(uint64_t v, uint64_t *s_w) {
+    const uint32_t wave = threadIdx.x >> 6, l = lane_id();
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint64_t u = __shfl_xor(v, o);
+        v = u > v ? u : v;
+    }
+    if (l == 0) s_w[wave] = v;
+    __syncthreads();
+    uint64_t m = 0;
+#pragma unroll
+    for (int w = 0; w < PLAN_NT / 64; w++) m = s_w[w] > m ? s_w[w] : m;
+    __syncthreads();
+    return m;
+}
+
 __global__ __launch_bounds__(PLAN_NT) void k_plan(TgtDev *__restrict__ tg, BatchDev *__restrict__ B,
                                                  PlanArgs A) {
     __shared__ uint64_t s_w[PLAN_NT / 64];
@@ -1111,22 +1128,44 @@ __global__ __launch_bounds__(PLAN_NT) void k_plan(TgtDev *__restrict__ tg, Batch
             T.digest = nullptr;
         }
     }
-    // two cref queues: whole-panel targets (transpose kernel) and the others
+    // two cref queues: whole-panel targets (transpose kernel) and the others.
+    // Block-major order (block b of every target before block b+1) when it
+    // costs few empty items: targets with similar windows then read the same
+    // panel rows at the same time, from the cache.
     const bool idk = fit && A.ident_ok && (tg[t].flags & TF_IDENTITY);
-    uint64_t nb_tot, nb_id;
-    const uint64_t bp = block_excl_scan64(fit && !idk ? nblk : 0u, s_w, nb_tot);
+    const bool gk = fit && !idk;
+    uint64_t nb_tot, nb_id, ng, ni, mxg, mxi;
+    const uint64_t bp = block_excl_scan64(gk ? nblk : 0u, s_w, nb_tot);
     const uint64_t bpi = block_excl_scan64(idk ? nblk : 0u, s_w, nb_id);
+    const uint64_t pg = block_excl_scan64(gk ? 1u : 0u, s_w, ng);
+    const uint64_t pi = block_excl_scan64(idk ? 1u : 0u, s_w, ni);
+    mxg = block_max64(gk ? nblk : 0u, s_w);
+    mxi = block_max64(idk ? nblk : 0u, s_w);
+    const bool rrg = ng > 1 && ng * mxg * 4 <= nb_tot * 5;
+    const bool rri = ni > 1 && ni * mxi * 4 <= nb_id * 5;
     if (valid) {
-        A.blockpref[t] = (uint32_t)bp;
-        A.blockpref_id[t] = (uint32_t)bpi;
+        if (rrg) {
+            if (gk) A.blockpref[pg] = t;
+        } else {
+            A.blockpref[t] = (uint32_t)bp;
+        }
+        if (rri) {
+            if (idk) A.blockpref_id[pi] = t;
+        } else {
+            A.blockpref_id[t] = (uint32_t)bpi;
+        }
     }
     uint64_t nfit;
     (void)block_excl_scan64(fit ? 1u : 0u, s_w, nfit);
     if (t == 0) {
-        A.blockpref[n] = (uint32_t)nb_tot;
-        A.blockpref_id[n] = (uint32_t)nb_id;
-        B->ncref = (uint32_t)nb_tot;
-        B->ncref_id = (uint32_t)nb_id;
+        if (!rrg) A.blockpref[n] = (uint32_t)nb_tot;
+        if (!rri) A.blockpref_id[n] = (uint32_t)nb_id;
+        B->ncref = (uint32_t)(rrg ? ng * mxg : nb_tot);
+        B->ncref_id = (uint32_t)(rri ? ni * mxi : nb_id);
+        B->rr[0] = rrg ? 1u : 0u;
+        B->rr[1] = rri ? 1u : 0u;
+        B->nrr[0] = (uint32_t)ng;
+        B->nrr[1] = (uint32_t)ni;
         B->npbwt = 2 * (uint32_t)nfit;
         B->nfit = (uint32_t)nfit;
         B->cref_ctr = 0;
@@ -1162,28 +1201,39 @@ __global__ __launch_bounds__(PLAN_NT) void k_plan(TgtDev *__restrict__ tg, Batch
     }
 }
 
-// next cref block from a queue: (block index i, its target t); i >= nq ends
-__device__ __forceinline__ void cref_pop(uint32_t *ctr, uint32_t nq, uint32_t n,
+// next cref block from a queue: queue position i (i >= nq ends), target t,
+// block b of t; b = ~0u for an empty item of a block-major queue
+__device__ __forceinline__ void cref_pop(uint32_t *ctr, uint32_t nq, uint32_t n, uint32_t rr,
+                                         uint32_t nrr, const TgtDev *__restrict__ tg,
                                          const uint32_t *__restrict__ pref, uint32_t &i,
-                                         uint32_t &t) {
-    __shared__ uint32_t s_i, s_t;
+                                         uint32_t &t, uint32_t &b) {
+    __shared__ uint32_t s_i, s_t, s_b;
     if (threadIdx.x == 0) {
         const uint32_t ii = atomicAdd(ctr, 1u);
-        uint32_t tt = 0;
-        if (ii < nq) {  // last target whose first block is <= ii (it owns block ii)
-            uint32_t lo = 0, hi = n;
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (pref[mid] <= ii) lo = mid; else hi = mid;
+        uint32_t tt = 0, bb = 0;
+        if (ii < nq) {
+            if (rr) {  // block-major: block ii / nrr of the (ii % nrr)-th listed target
+                tt = pref[ii % nrr];
+                bb = ii / nrr;
+                if (bb >= (tg[tt].Mp + 15) / 16) bb = ~0u;
+            } else {  // last target whose first block is <= ii (it owns block ii)
+                uint32_t lo = 0, hi = n;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (pref[mid] <= ii) lo = mid; else hi = mid;
+                }
+                tt = lo;
+                bb = ii - pref[lo];
             }
-            tt = lo;
         }
         s_i = ii;
         s_t = tt;
+        s_b = bb;
     }
     __syncthreads();
     i = s_i;
     t = s_t;
+    b = s_b;
     __syncthreads();
 }
 
@@ -1193,12 +1243,12 @@ __global__ __launch_bounds__(CREF_NT) void k_cref_persist(PanelDev P, const TgtD
                                                          const uint32_t *__restrict__ blockpref) {
     extern __shared__ uint64_t srow[];
     __shared__ uint32_t cnt[16];
-    const uint32_t nq = B->ncref, n = B->n;
+    const uint32_t nq = B->ncref, n = B->n, rr = B->rr[0], nrr = B->nrr[0];
     for (;;) {
-        uint32_t i, t;
-        cref_pop(&B->cref_ctr, nq, n, blockpref, i, t);
+        uint32_t i, t, b;
+        cref_pop(&B->cref_ctr, nq, n, rr, nrr, tg, blockpref, i, t, b);
         if (i >= nq) break;
-        cref_block_gather(P, tg[t], i - blockpref[t], srow, cnt);
+        if (b != ~0u) cref_block_gather(P, tg[t], b, srow, cnt);
     }
 }
 
@@ -1210,12 +1260,12 @@ __global__ __launch_bounds__(CREF_NT, 4) void k_cref_persist_ident(PanelDev P,
                                                                const uint32_t *__restrict__ blockpref) {
     extern __shared__ uint64_t srow[];
     __shared__ uint32_t cnt[16];
-    const uint32_t nq = B->ncref_id, n = B->n;
+    const uint32_t nq = B->ncref_id, n = B->n, rr = B->rr[1], nrr = B->nrr[1];
     for (;;) {
-        uint32_t i, t;
-        cref_pop(&B->cref_id_ctr, nq, n, blockpref, i, t);
+        uint32_t i, t, b;
+        cref_pop(&B->cref_id_ctr, nq, n, rr, nrr, tg, blockpref, i, t, b);
         if (i >= nq) break;
-        cref_block_ident<GPT>(P, tg[t], i - blockpref[t], srow, cnt);
+        if (b != ~0u) cref_block_ident<GPT>(P, tg[t], b, srow, cnt);
     }
 }
 
