@@ -83,6 +83,11 @@ int fail(int code, const std::string &msg) {
     } while (0)
 
 inline uint64_t unit_words(uint64_t nbits) { return ((nbits + 511) / 512) * 8; }
+inline uint32_t batch_layout(uint32_t flags) {
+    return (flags & EIMP_BATCH_COMPACT) ? LAYOUT_COMPACT
+         : (flags & EIMP_BATCH_RAW)     ? LAYOUT_RAW
+                                        : LAYOUT_GPU;
+}
 inline uint64_t round_up(uint64_t n, uint64_t m) { return (n + m - 1) / m * m; }
 
 // Grow-only device buffer
@@ -289,8 +294,8 @@ int ensure_pmono(eimp_engine *e) {
 }
 
 // Phase B for targets whose phase-A fields (bh, splitsites, S) are in place.
-// Sizes outputs, builds cref + PBWT.  rs_mode: 0 GPU layout, 1 compact.
-int run_build(eimp_engine *e, uint32_t n, bool digest, bool compact) {
+// Sizes outputs, builds cref + PBWT in the given output layout (LAYOUT_*).
+int run_build(eimp_engine *e, uint32_t n, bool digest, uint32_t layout) {
     std::vector<TgtDev> &tg = e->h_tg;
     uint64_t cref_bytes = 0, c0_bytes = 0, out_bytes = 0;
     std::vector<uint64_t> cref_off(n), c0_off(n), out_off(n);
@@ -301,7 +306,8 @@ int run_build(eimp_engine *e, uint32_t n, bool digest, bool compact) {
         TgtDev &T = tg[t];
         T.Mp = 2 * T.S + 1;
         T.Kpad = (uint32_t)round_up(T.K, 64);
-        T.RS = compact ? (uint32_t)(2 * ((T.K + 31) / 32)) : (uint32_t)(4 * unit_words(T.K));
+        T.RS = row_stride(T.K, layout);
+        if (layout == LAYOUT_RAW) T.flags |= TF_RAW;
         const uint64_t nblk = (T.Mp + 15) / 16;
         cref_off[t] = cref_bytes;
         cref_bytes += round_up(nblk * T.Kpad * 2, 256);
@@ -476,7 +482,7 @@ int stream_launch_sequence(eimp_stream *s) {
     A.blockpref_id = s->d_blockpref.as<uint32_t>() + s->nmax + 1;
     A.ident_ok = cref_ident_ok(e->pd) ? 1u : 0u;
     A.pitems = s->d_pitems.as<PbwtItem>();
-    A.compact = (s->flags & EIMP_BATCH_COMPACT) ? 1u : 0u;
+    A.layout = batch_layout(s->flags);
     A.digest = (s->flags & EIMP_BATCH_DIGEST) ? 1u : 0u;
     // timed contexts record an event before each kernel class (the planner
     // counts with the cref class, the count0 digest with the PBWT class)
@@ -745,6 +751,8 @@ int eimp_engine_run_batch(eimp_engine *e, size_t n, const eimp_target *targets, 
     if (!targets || !results) return fail(EIMP_ERR_INVALID, "null targets/results");
     if (!e->have_cM) return fail(EIMP_ERR_STATE, "genetic positions not set (eimp_engine_set_panel_extras)");
     if (n > (1u << 24)) return fail(EIMP_ERR_UNSUPPORTED, "too many targets in one batch");
+    if ((flags & EIMP_BATCH_COMPACT) && (flags & EIMP_BATCH_RAW))
+        return fail(EIMP_ERR_INVALID, "EIMP_BATCH_COMPACT and EIMP_BATCH_RAW exclude each other");
     CHK(set_device(e));
     const uint32_t nt = (uint32_t)n;
     const uint64_t Nw = unit_words(e->Nad);
@@ -864,7 +872,7 @@ int eimp_engine_run_batch(eimp_engine *e, size_t n, const eimp_target *targets, 
     for (uint32_t t = 0; t < nt; t++) e->h_tg[t].S = e->h_stats[4 * t];
     // ---- phase B
     const bool digest = (flags & EIMP_BATCH_DIGEST) != 0;
-    CHK(run_build(e, nt, digest, (flags & EIMP_BATCH_COMPACT) != 0));
+    CHK(run_build(e, nt, digest, batch_layout(flags)));
     std::vector<uint64_t> dig(digest ? 2 * (size_t)nt : 0);
     if (digest)
         HIPCHK(hipMemcpyAsync(dig.data(), e->d_digest.p, (size_t)nt * 16, hipMemcpyDeviceToHost,
@@ -951,7 +959,7 @@ int eimp_engine_run_cref_pbwt(eimp_engine *e, size_t n, const uint64_t *const *t
                                               ": crefTOffsets disagree with the split bitmap");
         e->h_tg[t].S = S;
     }
-    CHK(run_build(e, nt, false, false));
+    CHK(run_build(e, nt, false, LAYOUT_GPU));
     // results to the caller's host buffers, as src/GPUEngine.cu:182-211
     for (uint32_t t = 0; t < nt; t++) {
         const TgtDev &T = e->h_tg[t];
@@ -1038,8 +1046,11 @@ int eimp_stream_create(eimp_engine *e, size_t max_targets, size_t max_window, si
     if (max_window < 1 || max_window > e->M) return fail(EIMP_ERR_INVALID, "max_window must be in [1, M]");
     if (max_K < 1) return fail(EIMP_ERR_INVALID, "max_K < 1");
     if (pool_bytes < 256) return fail(EIMP_ERR_INVALID, "pool_bytes too small");
-    if (flags & ~(EIMP_BATCH_DIGEST | EIMP_BATCH_COMPACT | EIMP_STREAM_NO_GRAPH | EIMP_STREAM_TIMED))
+    if (flags & ~(EIMP_BATCH_DIGEST | EIMP_BATCH_COMPACT | EIMP_BATCH_RAW | EIMP_STREAM_NO_GRAPH |
+                  EIMP_STREAM_TIMED))
         return fail(EIMP_ERR_INVALID, "unknown stream flags");
+    if ((flags & EIMP_BATCH_COMPACT) && (flags & EIMP_BATCH_RAW))
+        return fail(EIMP_ERR_INVALID, "EIMP_BATCH_COMPACT and EIMP_BATCH_RAW exclude each other");
     CHK(set_device(e));
     eimp_stream *s = new eimp_stream();
     s->e = e;
@@ -1081,6 +1092,7 @@ int eimp_stream_submit(eimp_stream *s, size_t n, const eimp_target *targets, flo
         const eimp_target &x = targets[t];
         TgtDev &T = tab[t];
         CHK(target_inputs(e, x, t, cMmaxSplit, T));
+        if (batch_layout(s->flags) == LAYOUT_RAW) T.flags |= TF_RAW;
         if (x.W > s->Wmax) return fail(EIMP_ERR_INVALID, "target " + std::to_string(t) + ": window beyond max_window");
         if (T.K > s->Kmax) return fail(EIMP_ERR_INVALID, "target " + std::to_string(t) + ": K beyond max_K");
         const uint64_t w = 2 * unit_words(x.W);

@@ -20,7 +20,20 @@ constexpr uint32_t kExcluded = 0xFFFFFFFFu;
 enum : uint32_t {
     TF_SELECT = 1u,    // K < Ncorr: top-K selection by (nerr, idx)
     TF_IDENTITY = 2u,  // bestHaps == 0..N_ad-1 (whole panel, no exclusions)
+    TF_RAW = 4u,       // output rows hold data words only (raw layout)
 };
+
+// Output row layouts and their stride in uint32 per stored row
+enum : uint32_t {
+    LAYOUT_GPU = 0u,      // 4*Kwords: (data, cumulative zeros) pairs, tail pairs zero
+    LAYOUT_COMPACT = 1u,  // 2*ceil(K/32) pairs, the CPU layout
+    LAYOUT_RAW = 2u,      // roundToMultiple(K,512)/32 data words, tail words all ones
+};
+__host__ __device__ inline uint32_t row_stride(uint32_t K, uint32_t layout) {
+    return layout == LAYOUT_COMPACT ? 2 * ((K + 31) / 32)
+         : layout == LAYOUT_RAW     ? (K + 511) / 512 * 16
+                                    : 4 * ((K + 511) / 512 * 8);
+}
 
 // Per-target record in device memory.  Filled by the host in two stages:
 // the prep fields before phase A, the build fields (Mp, offsets) after the
@@ -96,7 +109,7 @@ struct PlanArgs {
     uint32_t *blockpref;     // nmax + 1 prefix sums of cref blocks (gather targets)
     uint32_t *blockpref_id;  // nmax + 1 prefix sums of cref blocks (identity targets)
     PbwtItem *pitems;        // 2 * nmax
-    uint32_t compact;        // rows of 2*ceil(K/32) instead of 4*Kwords
+    uint32_t layout;         // LAYOUT_*
     uint32_t digest;
     uint32_t ident_ok;       // identity targets take the transpose kernel
     uint32_t pad;

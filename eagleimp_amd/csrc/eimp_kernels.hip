@@ -896,6 +896,7 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
     const uint64_t gbase = rev ? (uint64_t)Mp * KW : 0ull;
     uint64_t dsum = 0;
     const uint32_t nslots = RS / 4;        // 4-word slots of a stored row
+    const bool raw = (T.flags & TF_RAW) != 0;
     const uint32_t och = ch0 + ln;         // chunk stored by this lane
     const bool ostore = ln < (uint32_t)NS && och < nch;
     const uint32_t ln4 = 4u * ln;
@@ -965,16 +966,21 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < NS; c++) p[c] = pay[(ch0 + (uint32_t)c) * 64 + ln];
-        // the row: chunk ch holds pairs 2ch and 2ch+1
+        // the row: chunk ch holds pairs 2ch and 2ch+1 (raw layout: words 2ch, 2ch+1)
         uint32_t *__restrict__ row = outbase + (uint64_t)step * RS;
         if (ostore) {
             const uint32_t o1 = zbv + 32u - (uint32_t)__popc(olo);
             const uint32_t o3 = o1 + 32u - (uint32_t)__popc(ohi);
             const bool second = 2 * och + 1 < npairs;
-            uint2 *dst = (uint2 *)(row + 4 * och);
-            dst[0] = make_uint2(olo, o1);
-            if (second) dst[1] = make_uint2(ohi, o3);
-            else if (4 * och + 2 < RS) dst[1] = make_uint2(0u, 0u);
+            if (raw) {
+                // positions >= K are all ones: a word past the logical row is ~0
+                *(uint2 *)(row + 2 * och) = make_uint2(olo, ohi);
+            } else {
+                uint2 *dst = (uint2 *)(row + 4 * och);
+                dst[0] = make_uint2(olo, o1);
+                if (second) dst[1] = make_uint2(ohi, o3);
+                else if (4 * och + 2 < RS) dst[1] = make_uint2(0u, 0u);
+            }
             if (dig) {
                 // rotl(v ^ key, g mod 32) == rotr by -g: one v_alignbit each
                 dsum += __builtin_amdgcn_alignbit(olo ^ dkey, olo ^ dkey, dneg);
@@ -989,9 +995,14 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
         }
         dkey += dkey_step;
         dneg -= KW;
-        // zero slots after the logical row
-        const uint32_t tail = nch + threadIdx.x;
-        if (tail < nslots) ((uint4 *)row)[tail] = make_uint4(0u, 0u, 0u, 0u);
+        // slots after the logical row: zero pairs, or all-ones words (raw)
+        if (raw) {
+            const uint32_t tail = nch + threadIdx.x;  // pairs of words
+            if (tail < RS / 2) ((uint2 *)row)[tail] = make_uint2(~0u, ~0u);
+        } else {
+            const uint32_t tail = nch + threadIdx.x;
+            if (tail < nslots) ((uint4 *)row)[tail] = make_uint4(0u, 0u, 0u, 0u);
+        }
     }
     if (dig) {
         dsum = wave_sum64(dsum);
@@ -1097,7 +1108,7 @@ __global__ __launch_bounds__(PLAN_NT) void k_plan(TgtDev *__restrict__ tg, Batch
         K = T.K;
         Mp = 2 * S + 1;
         Kpad = (K + 63) / 64 * 64;
-        RS = A.compact ? 2 * ((K + 31) / 32) : 4 * (((K + 511) / 512) * 8);
+        RS = row_stride(K, A.layout);
         nblk = (Mp + 15) / 16;
         cb = region_cref_bytes(Mp, Kpad);
         c0b = region_c0_bytes(Mp);

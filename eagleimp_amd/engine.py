@@ -16,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libeimp.so")
 
 EIMP_BATCH_DIGEST = 1
+EIMP_BATCH_RAW = 2
 EIMP_BATCH_COMPACT = 4
 EIMP_STREAM_NO_GRAPH = 8
 EIMP_STREAM_TIMED = 16
@@ -205,7 +206,10 @@ class Engine:
 
     # -- batched device pre-stage
     def run_batch(self, gts, s0s, Ws, Ks, ignore=None, cMmaxSplit=0.125, digest=True,
-                  compact=False) -> BatchResult:
+                  compact=False, raw=False) -> BatchResult:
+        """compact: rows of 2*ceil(K/32) (CPU layout); raw: data words only in
+        rows of roundToMultiple(K,512)/32 (FPGA handler layout); default: the
+        drop-in GPU layout."""
         n = len(gts)
         self._gts = [np.ascontiguousarray(g, dtype=np.uint64) for g in gts]
         ts = (EimpTarget * n)()
@@ -213,11 +217,12 @@ class Engine:
             ts[i] = EimpTarget(self._gts[i].ctypes.data, int(s0s[i]), int(Ws[i]), int(Ks[i]),
                                -1 if ignore is None else int(ignore[i]))
         rs = (EimpResult * n)()
-        flags = (EIMP_BATCH_DIGEST if digest else 0) | (EIMP_BATCH_COMPACT if compact else 0)
+        flags = ((EIMP_BATCH_DIGEST if digest else 0) | (EIMP_BATCH_COMPACT if compact else 0)
+                 | (EIMP_BATCH_RAW if raw else 0))
         _check(lib().eimp_engine_run_batch(self._h, n, ts, C.c_float(cMmaxSplit), flags, rs))
         arr = np.frombuffer(rs, dtype=np.uint64).reshape(n, 6)
         self._last = arr.copy()
-        self._last_compact = compact
+        self._last_layout = layout_name(compact, raw)
         self._last_W = [int(w) for w in Ws]
         return BatchResult(S=arr[:, 0].copy(), Mp=arr[:, 1].copy(), ncalls=arr[:, 2].copy(),
                            K=arr[:, 3].copy(), dig_pbwt=arr[:, 4].copy(),
@@ -226,7 +231,7 @@ class Engine:
     def copy_target(self, t, cref=False) -> dict:
         S, Mp, _, K = (int(x) for x in self._last[t, :4])
         return _copy_out(lib().eimp_engine_copy_target, self._h, t, S, Mp, K, self._last_W[t],
-                         self._last_compact, self.Nad, cref)
+                         self._last_layout, self.Nad, cref)
 
     def mem_info(self):
         """(free, total) device bytes."""
@@ -235,9 +240,9 @@ class Engine:
         return f.value, t.value
 
     def stream(self, max_targets, max_window, max_K, pool_bytes, digest=True, compact=False,
-               graph=True, timed=False) -> "Stream":
+               graph=True, timed=False, raw=False) -> "Stream":
         return Stream(self, max_targets, max_window, max_K, pool_bytes, digest, compact, graph,
-                      timed)
+                      timed, raw)
 
     # -- measurement
     def set_timing(self, on=True):
@@ -256,8 +261,23 @@ class Engine:
         _check(lib().eimp_engine_set_pbwt_variant(self._h, v))
 
 
-def _copy_out(fn, h, t, S, Mp, K, W, compact, Nad, cref) -> dict:
-    rs = 2 * ((K + 31) // 32) if compact else 4 * unit_words(K)
+def layout_name(compact=False, raw=False) -> str:
+    if compact and raw:
+        raise ValueError("compact and raw layouts exclude each other")
+    return "compact" if compact else ("raw" if raw else "gpu")
+
+
+def row_stride(K: int, layout: str) -> int:
+    """uint32 words per stored PBWT row in an output layout."""
+    if layout == "compact":
+        return 2 * ((K + 31) // 32)
+    if layout == "raw":
+        return (K + 511) // 512 * 16
+    return 4 * unit_words(K)
+
+
+def _copy_out(fn, h, t, S, Mp, K, W, layout, Nad, cref) -> dict:
+    rs = row_stride(K, layout)
     out = dict(
         besthaps=np.zeros(K, np.uint32),
         bhflags=np.zeros(unit_words(Nad), np.uint64),
@@ -287,12 +307,13 @@ class Stream:
     kernel classes, read with kernel_ms())."""
 
     def __init__(self, eng: Engine, max_targets, max_window, max_K, pool_bytes, digest=True,
-                 compact=False, graph=True, timed=False):
+                 compact=False, graph=True, timed=False, raw=False):
         self.eng = eng
         self._h = _P()
-        self.compact = compact
+        self.layout = layout_name(compact, raw)
         self.digest = digest
         flags = ((EIMP_BATCH_DIGEST if digest else 0) | (EIMP_BATCH_COMPACT if compact else 0)
+                 | (EIMP_BATCH_RAW if raw else 0)
                  | (0 if graph else EIMP_STREAM_NO_GRAPH) | (EIMP_STREAM_TIMED if timed else 0))
         _check(lib().eimp_stream_create(eng._h, max_targets, max_window, max_K, pool_bytes, flags,
                                         C.byref(self._h)))
@@ -358,4 +379,4 @@ class Stream:
     def copy_target(self, t, cref=False) -> dict:
         S, Mp, _, K = (int(x) for x in self._last[t, :4])
         return _copy_out(lib().eimp_stream_copy_target, self._h, t, S, Mp, K, self._W[t],
-                         self.compact, self.eng.Nad, cref)
+                         self.layout, self.eng.Nad, cref)

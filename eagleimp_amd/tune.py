@@ -48,14 +48,15 @@ def _store(db):
         pass
 
 
-def region_bytes(Mp, K, compact=False):
+def region_bytes(Mp, K, layout="gpu"):
     """Bytes a target takes in a context's output pool (cref blocks, count0,
     forward + backward PBWT rows, each rounded to 256 bytes)."""
     Mp = np.asarray(Mp, np.int64)
     K = np.asarray(K, np.int64)
     up = lambda x: (x + 255) // 256 * 256  # noqa: E731
     kpad = (K + 63) // 64 * 64
-    rs = 2 * ((K + 31) // 32) if compact else 4 * ((K + 511) // 512) * 8
+    rs = {"compact": 2 * ((K + 31) // 32), "raw": (K + 511) // 512 * 16}.get(
+        layout, 4 * ((K + 511) // 512) * 8)
     return up((Mp + 15) // 16 * kpad * 2) + up(Mp * 4) + up(2 * Mp * rs * 4)
 
 

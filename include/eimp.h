@@ -117,6 +117,12 @@ typedef struct eimp_target {
 
 /* Batch flags */
 #define EIMP_BATCH_DIGEST 1u      /* compute per-target output digests */
+#define EIMP_BATCH_RAW 2u         /* rows of roundToMultiple(K,512)/32 data words
+                                     only, bits past K and words past the row one:
+                                     the FPGA handler's raw buffer, which
+                                     PBWT(cpbwt, K, Kwords, M, ntarget) turns into
+                                     data/offset pairs (src/PBWT.cpp:72-110,
+                                     src/FPGAHandler.cpp:896-900) */
 #define EIMP_BATCH_COMPACT 4u     /* rows of 2*ceil(K/32) uint32 (CPU layout,
                                      src/PBWT.cpp:36) instead of 4*Kwords */
 
@@ -166,9 +172,9 @@ typedef struct eimp_stream eimp_stream;
  * fixed device buffers for up to max_targets (<= 1024) targets with windows of
  * up to max_window sites and K up to max_K (after clamping), an output pool of
  * pool_bytes, and the captured launch sequence.  flags: EIMP_BATCH_DIGEST,
- * EIMP_BATCH_COMPACT, EIMP_STREAM_NO_GRAPH.  The panel is shared with the
- * engine and its other contexts; each context is used by one host thread at
- * a time. */
+ * EIMP_BATCH_COMPACT or EIMP_BATCH_RAW, EIMP_STREAM_NO_GRAPH,
+ * EIMP_STREAM_TIMED.  The panel is shared with the engine and its other
+ * contexts; each context is used by one host thread at a time. */
 int eimp_stream_create(eimp_engine *engine, size_t max_targets, size_t max_window, size_t max_K,
                        size_t pool_bytes, uint32_t flags, eimp_stream **stream);
 

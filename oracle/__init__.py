@@ -98,6 +98,10 @@ def ref():
         R.ref_build_pbwt.argtypes = [_P, _U64, _U64, _U64, _P, _P, _U64, _P, _P]
         R.ref_genotype_compare.restype = C.c_int
         R.ref_genotype_compare.argtypes = [_P, _U64, _P, _U64, _P, _U64, _P]
+        R.ref_ingest_raw.restype = C.c_int
+        R.ref_ingest_raw.argtypes = [_P, _U64, _U64, _U64, _P, _P]
+        R.ref_build_ref_pbwt.restype = C.c_int
+        R.ref_build_ref_pbwt.argtypes = [_P, _U64, _U64, _U64, _P, _P, _P]
         _REF = R
     return _REF
 
@@ -269,3 +273,38 @@ def ref_genotype_compare(codes, hap, hap_len):
     if n < 0:
         raise RuntimeError("ref_genotype_compare failed")
     return words[:n], int(nerr[0])
+
+
+def raw_rows(pbwt_rows, K):
+    """The raw (FPGA handler) layout of compact rows: the data words of each
+    row, padded with all-ones words to roundToMultiple(K,512)/32 words."""
+    nw = (K + 31) // 32
+    kr = (K + 511) // 512 * 16
+    out = np.full((pbwt_rows.shape[0], kr), 0xFFFFFFFF, np.uint32)
+    out[:, :nw] = pbwt_rows[:, 0:2 * nw:2]
+    return out
+
+
+def ref_ingest_raw(raw, K):
+    """The reference's PBWT(uint32_t*, K, Kwords, M) on a raw buffer: (rows with
+    inserted zero counts, count0)."""
+    raw = np.ascontiguousarray(raw, np.uint32)
+    M = raw.shape[0] // 2
+    out = np.zeros((2 * M, 2 * raw.shape[1]), np.uint32)
+    c0 = np.zeros(M, np.int32)
+    if ref().ref_ingest_raw(_p(raw), K, raw.shape[1], M, _p(out), _p(c0)) != 0:
+        raise RuntimeError("ref_ingest_raw failed")
+    return out, c0
+
+
+def ref_build_ref_pbwt(rows, K, absperm=True):
+    """The reference's refPBWT over site rows (uint64, bit h = haplotype h):
+    (rows of 2*ceil(K/32) uint32, count0, absolute permutations (M+1) x K)."""
+    rows = np.ascontiguousarray(rows, np.uint64)
+    M = rows.shape[0]
+    out = np.zeros((M, 2 * ((K + 31) // 32)), np.uint32)
+    c0 = np.zeros(M, np.int32)
+    ap = np.zeros((M + 1, K), np.int32) if absperm else None
+    if ref().ref_build_ref_pbwt(_p(rows), rows.shape[1], K, M, _p(out), _p(c0), _p(ap)) != 0:
+        raise RuntimeError("ref_build_ref_pbwt failed")
+    return out, c0, ap

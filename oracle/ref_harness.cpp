@@ -102,4 +102,51 @@ int ref_genotype_compare(const uint8_t *g, uint64_t W, const uint64_t *hap, uint
     return (int)d.size();
 }
 
+// Ingests a raw cPBWT buffer (2*M rows of kwords_raw data words, the FPGA
+// handler's layout, src/FPGAHandler.cpp:896-900) with the reference's
+// PBWT(uint32_t*, K, Kwords, M, ntarget) (src/PBWT.cpp:72-110), which inserts
+// the cumulative zero counts; returns its rows (2*M x 2*kwords_raw uint32)
+// and count0 (M).
+int ref_ingest_raw(const uint32_t *raw, uint64_t K, uint64_t kwords_raw, uint64_t M, uint32_t *out,
+                   int32_t *count0) {
+    std::vector<uint32_t> buf(raw, raw + 2 * M * kwords_raw);  // the constructor copies it
+    size_t id = g_next_id++;
+    PBWT pbwt(buf.data(), K, kwords_raw, M, id);
+    const uint32_t *data = findAlloc(std::string("comprPBWT_t") + std::to_string(id));
+    if (!data) return -1;
+    std::memcpy(out, data, 2 * M * 2 * kwords_raw * sizeof(uint32_t));
+    for (size_t m = 0; m < M; m++) count0[m] = pbwt.getCount0(m);
+    return 0;
+}
+
+// The reference panel PBWT (refPBWT: forward only, absolute permutations kept,
+// src/PBWT.cpp:53-71,131-153) over M site rows of K haplotypes (rows of
+// row_stride uint64 words).  out: M rows of 2*ceil(K/32) uint32; count0: M;
+// absperm: (M+1) x K int32 (row m = order before site m), or NULL.
+int ref_build_ref_pbwt(const uint64_t *rows, uint64_t row_stride, uint64_t K, uint64_t M,
+                       uint32_t *out, int32_t *count0, int32_t *absperm) {
+    size_t capacity = roundUp(K, UNITWORDS * 64) / 8;
+    std::vector<uint64_t> buf(M * capacity / 8, 0ull);
+    for (size_t m = 0; m < M; m++)
+        std::memcpy(&buf[m * capacity / 8], rows + m * row_stride, ((K + 63) / 64) * 8);
+    RingBuffer<BooleanVector> refT(M, M);
+    uint64_t *cur = buf.data();
+    for (auto &row : refT) {
+        row.setData(cur, capacity, K);
+        cur += capacity / 8;
+    }
+    size_t id = g_next_id++;
+    PBWT pbwt(&refT, true, id);
+    const uint32_t *data = findAlloc(std::string("comprPBWT_t") + std::to_string(id));
+    if (!data) return -1;
+    std::memcpy(out, data, M * 2 * ((K + 31) / 32) * sizeof(uint32_t));
+    for (size_t m = 0; m < M; m++) count0[m] = pbwt.getCount0(m);
+    if (absperm)
+        for (size_t m = 0; m <= M; m++) {
+            const auto &o = pbwt.getSortOrder(m);
+            std::memcpy(absperm + m * K, o.data(), K * sizeof(int32_t));
+        }
+    return 0;
+}
+
 }  // extern "C"

@@ -14,7 +14,12 @@ def panel_engine(panel, Ncorr=None, haploid=None, give_hap=False):
     return eng
 
 
-def check_target(ref, got, K, compact, what):
+def check_target(ref, got, K, layout, what):
+    """layout: "gpu", "compact", "raw" (True / False stand for compact / gpu)."""
+    if layout is True:
+        layout = "compact"
+    elif layout is False:
+        layout = "gpu"
     assert np.array_equal(got["besthaps"], ref["besthaps"]), f"{what}: besthaps"
     assert np.array_equal(got["bhflags"], ref["flags"]), f"{what}: best-hap flags"
     assert np.array_equal(got["splitsites"], ref["splitsites"]), f"{what}: split sites"
@@ -25,8 +30,10 @@ def check_target(ref, got, K, compact, what):
     if "cref" in got:
         assert np.array_equal(got["cref"], ref["cref"]), f"{what}: condensed reference"
     KW = oracle.pbwt_row_words(K)
-    if compact:
+    if layout == "compact":
         assert np.array_equal(got["pbwt"], ref["pbwt"]), f"{what}: cPBWT (compact)"
+    elif layout == "raw":
+        assert np.array_equal(got["pbwt"], oracle.raw_rows(ref["pbwt"], K)), f"{what}: cPBWT (raw)"
     else:
         assert np.array_equal(got["pbwt"][:, :KW], ref["pbwt"]), f"{what}: cPBWT logical words"
         assert not got["pbwt"][:, KW:].any(), f"{what}: cPBWT padding pairs not zero"

@@ -19,9 +19,9 @@ pytestmark = pytest.mark.gpu
 
 
 def _run_and_compare(panel, pv, eng, gts, s0s, Ws, Ks, ignore=None, compact=False, cref=True,
-                     tag=""):
+                     tag="", raw=False):
     say(tag, "targets", len(gts), "K", list(Ks), "W", list(Ws), "s0", list(s0s))
-    res = eng.run_batch(gts, s0s, Ws, Ks, ignore=ignore, compact=compact, digest=True)
+    res = eng.run_batch(gts, s0s, Ws, Ks, ignore=ignore, compact=compact, raw=raw, digest=True)
     for t in range(len(gts)):
         r = oracle.process_target(pv if not callable(pv) else pv(t), gts[t], s0s[t], Ws[t], Ks[t])
         what = f"{tag} target {t} (K={Ks[t]}, W={Ws[t]}, s0={s0s[t]})"
@@ -29,7 +29,7 @@ def _run_and_compare(panel, pv, eng, gts, s0s, Ws, Ks, ignore=None, compact=Fals
         assert int(res.Mp[t]) == r["Mp"], f"{what}: Mp"
         assert int(res.ncalls[t]) == r["ncalls"], f"{what}: ncalls"
         got = eng.copy_target(t, cref=cref)
-        _check_target(r, got, int(r["K"]), compact, what)
+        _check_target(r, got, int(r["K"]), "raw" if raw else ("compact" if compact else "gpu"), what)
         assert int(res.dig_pbwt[t]) == r["dig_pbwt"], f"{what}: pbwt digest"
         assert int(res.dig_count0[t]) == r["dig_count0"], f"{what}: count0 digest"
     return res
@@ -68,6 +68,19 @@ def test_compact_layout_and_given_hap(built, small):
     tg = synth.make_targets(panel, 5, 6, kset=[50, 97, 300, 129], wmin=300, wmax=4000, miss=0.0,
                             miss_hi=0.05)
     _run_and_compare(panel, pv, eng, tg.gt, tg.s0, tg.W, tg.K, compact=True, tag="compact")
+    eng.close()
+
+
+def test_raw_layout(built, small):
+    """EIMP_BATCH_RAW: data words only, rows of roundToMultiple(K,512)/32 words
+    (the FPGA handler's buffer), K on both sides of a 512 boundary."""
+    panel = small
+    pv = oracle.PanelView(panel.hap, panel.site, panel.cM)
+    eng = _panel_engine(panel)
+    tg = synth.make_targets(panel, 7, 5, kset=[31, 64, 200, 299, 300], wmin=300, wmax=4000)
+    _run_and_compare(panel, pv, eng, tg.gt, tg.s0, tg.W, tg.K, raw=True, tag="raw")
+    with pytest.raises(EimpError):  # the C ABI rejects two layouts at once
+        eng.run_batch(tg.gt, tg.s0, tg.W, tg.K, compact=True, raw=True)
     eng.close()
 
 

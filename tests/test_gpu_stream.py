@@ -29,7 +29,7 @@ def _refs(pv, tg, idx):
                                   tg.K[t]) for t in idx]
 
 
-def _compare(st, res, refs, idx, compact, tag):
+def _compare(st, res, refs, idx, layout, tag):
     for j, t in enumerate(idx):
         r = refs[j]
         what = f"{tag} target {j}"
@@ -37,13 +37,13 @@ def _compare(st, res, refs, idx, compact, tag):
         assert int(res.Mp[j]) == r["Mp"], f"{what}: Mp"
         assert int(res.ncalls[j]) == r["ncalls"], f"{what}: ncalls"
         got = st.copy_target(j, cref=True)
-        check_target(r, got, int(r["K"]), compact, what)
+        check_target(r, got, int(r["K"]), layout, what)
         assert int(res.dig_pbwt[j]) == r["dig_pbwt"], f"{what}: pbwt digest"
         assert int(res.dig_count0[j]) == r["dig_count0"], f"{what}: count0 digest"
 
 
-@pytest.mark.parametrize("compact", [False, True])
-def test_stream_matches_oracle(built, small, compact):
+@pytest.mark.parametrize("layout", ["gpu", "compact", "raw"])
+def test_stream_matches_oracle(built, small, layout):
     """Two submissions in a row on one context (the second smaller), mixed
     windows and K including the whole-panel identity case."""
     panel = small
@@ -51,15 +51,15 @@ def test_stream_matches_oracle(built, small, compact):
     eng = panel_engine(panel)
     tg = synth.make_targets(panel, 41, 12, kset=[300, 64, 150, 299, 2, 257], wmin=60, wmax=4000,
                             miss=0.0, miss_hi=0.05)
-    st = eng.stream(16, 4000, 300, 256 * MB, compact=compact)
+    st = eng.stream(16, 4000, 300, 256 * MB, compact=layout == "compact", raw=layout == "raw")
     for idx in (list(range(12)), [3, 7, 1]):
-        say("stream compact", compact, "targets", idx, "K", [int(tg.K[t]) for t in idx])
+        say("stream layout", layout, "targets", idx, "K", [int(tg.K[t]) for t in idx])
         st.submit([tg.gt[t] for t in idx], [tg.s0[t] for t in idx], [tg.W[t] for t in idx],
                   [tg.K[t] for t in idx])
         res, nd, (kms, sms) = st.wait()
         assert nd == 0
         assert 0 <= kms <= sms
-        _compare(st, res, _refs(pv, tg, idx), idx, compact, f"stream {idx}")
+        _compare(st, res, _refs(pv, tg, idx), idx, layout, f"stream {idx}")
     st.close()
     eng.close()
 

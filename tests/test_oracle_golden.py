@@ -104,3 +104,23 @@ def test_oracle_pipeline_pbwt_against_reference(built):
         r = oracle.process_target(pv, tg.gt[t], tg.s0[t], tg.W[t], tg.K[t])
         ref, _ = oracle.ref_build_pbwt(r["cref"], int(r["K"]), r["count0"])
         assert np.array_equal(r["pbwt"], ref)
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference harness not built here")
+def test_raw_layout_ingested_by_reference(built):
+    """The raw layout (data words only, rows of roundToMultiple(K,512)/32
+    words, EIMP_BATCH_RAW) fed to the reference's own raw-buffer constructor
+    (src/PBWT.cpp:72-110) gives back the CPU-layout rows and count0."""
+    rng = np.random.default_rng(99)
+    for K, Mp in [(2, 5), (47, 31), (512, 7), (513, 4), (1531, 9)]:
+        bits = (rng.random((Mp, K)) < rng.random()).astype(np.uint8)
+        kw = synth.unit_words(K)
+        padded = np.zeros((Mp, kw * 64), np.uint8)
+        padded[:, :K] = bits
+        cref = np.packbits(padded, axis=1, bitorder="little").view(np.uint64)
+        c0 = (K - bits.sum(axis=1)).astype(np.int32)
+        rows = oracle.build_pbwt(cref, K, c0)
+        got, gc0 = oracle.ref_ingest_raw(oracle.raw_rows(rows, K), K)
+        nw = (K + 31) // 32
+        assert np.array_equal(got[:, :2 * nw], rows), (K, Mp)
+        assert np.array_equal(gc0, c0), (K, Mp)
