@@ -132,6 +132,62 @@ def serve_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def panel_pbwt_bench(args):
+    """The panel PBWT for imputation (eimp_engine_ref_pbwt: one forward PBWT
+    over all panel haplotypes with the absolute order after every site) on a
+    workload's panel, against the oracle's single-threaded restatement (the
+    reference builds it on one thread, src/PBWTImputer.cpp:110,119)."""
+    from eagleimp_amd import synth
+    from eagleimp_amd.engine import Engine
+    import oracle
+
+    cfg = dict(synth.WORKLOADS[args.workload])
+    panel = synth.make_panel(cfg.get("panel_seed", args.workload), cfg["N"], cfg["M"], cfg["F"], 1,
+                             beta_a=cfg["beta_a"])
+    N, M = panel.N, panel.M
+    eng = Engine(0)
+    eng.initialize(2, M, N, N, panel.site)
+    eng.set_panel_extras(panel.cM)
+    log(f"panel PBWT: N={N} M={M}")
+    eng.ref_pbwt(0, min(M, 1000))  # warm-up
+    t = time.perf_counter()
+    rows, c0, ap, dev_ms = eng.ref_pbwt(0, M, absperm=True)
+    wall = time.perf_counter() - t
+    log(f"device {dev_ms:.1f} ms, with copies of rows / count0 / {ap.nbytes / 1e9:.1f} GB of orders: "
+        f"{wall:.2f} s")
+    # the oracle on a prefix of the sites, checked against the device rows
+    t = time.perf_counter()
+    oracle.build_ref_pbwt(panel.site[:200], N)
+    per = (time.perf_counter() - t) / 200
+    n = int(max(200, min(M, args.cpu_seconds / max(per, 1e-9))))
+    t = time.perf_counter()
+    er, ec, ea = oracle.build_ref_pbwt(panel.site[:n], N)
+    cpu_s = time.perf_counter() - t
+    exact = bool(np.array_equal(er, rows[:n]) and np.array_equal(ec, c0[:n])
+                 and np.array_equal(ea, ap[:n + 1]))
+    log(f"oracle: {n} sites in {cpu_s:.1f} s; bit_exact={exact}")
+    steps = N * M
+    out = {
+        "metric": "panel PBWT for imputation: sites/s (forward rows + count0 + absolute orders)",
+        "value": round(M / (dev_ms / 1e3), 1), "unit": "sites/s", "n_gpus": 1, "steps": 1,
+        "warmup": 1, "ms_per_step": round(dev_ms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": cfg["name"] + "-panel", "N": N, "M": M},
+        "bit_exact": exact,
+        "element_steps_per_s": round(steps / (dev_ms / 1e3), 1),
+        "with_copies_sites_per_s": round(M / wall, 1),
+        "roofline": {"bound": "hbm", "kernel": "k_panel_pbwt",
+                     "achieved": round((M * (N * 4 + 8 * ((N + 31) // 32)) + M * N / 8) / (dev_ms / 1e3) / 1e9, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None,
+                     "note": "one workgroup (one CU): the site loop is sequential"},
+        "cpu_baseline": {"value": round(n / cpu_s, 1), "unit": "sites/s", "cores": 1,
+                         "kind": "port", "sample": f"first {n} sites, oracle restatement"},
+    }
+    out["roofline"]["frac"] = round(out["roofline"]["achieved"] / HBM_PEAK_GBS, 5)
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +217,8 @@ def main():
                          "the autotuner on a sample of the workload first")
     ap.add_argument("--graph-steps", type=int, default=2,
                     help="steps replayed through captured hipGraphs after the timed region")
+    ap.add_argument("--panel-pbwt", action="store_true",
+                    help="measure the panel PBWT for imputation on the workload's panel instead")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (scales HBM counters)")
     ap.add_argument("--rocpd-summary", nargs="+", metavar="DB",
@@ -190,6 +248,9 @@ def main():
 
     if args.workload == 5:
         serve_bench(args)
+        return
+    if args.panel_pbwt:
+        panel_pbwt_bench(args)
         return
 
     from eagleimp_amd import synth

@@ -50,6 +50,9 @@ hipError_t launch_digest_count0_dev(const TgtDev *tg, const uint32_t *d_n, uint3
 size_t cref_lds_bytes(const PanelDev &P, uint32_t maxK);
 hipError_t cref_persist_prepare(const PanelDev &P, uint32_t maxK, int *occ);
 bool cref_ident_ok(const PanelDev &P);
+hipError_t launch_panel_pbwt(const PanelDev &P, uint64_t m0, uint32_t nsites, uint16_t *blocks,
+                             uint32_t *rows, int32_t *c0out, int32_t *absperm, hipStream_t s);
+uint32_t panel_pbwt_max_k();
 hipError_t launch_cref_persist(const PanelDev &P, const TgtDev *tg, BatchDev *B,
                                const uint32_t *blockpref, const uint32_t *blockpref_id,
                                uint32_t maxK, const uint32_t *grid, hipStream_t s);
@@ -164,6 +167,7 @@ struct eimp_engine {
     // batch pools
     DevBuf d_tg, d_gt, d_nerr, d_bh, d_bhflags, d_mono, d_splitsites, d_splits, d_cMdiffs,
         d_stats, d_cref, d_count0, d_out, d_digest, d_tlist, d_citems, d_pitems, d_tmp;
+    DevBuf p_blocks, p_rows, p_c0, p_abs;  // panel PBWT
     HostBuf h_stage;
     std::vector<TgtDev> h_tg;
     std::vector<uint32_t> h_stats;
@@ -221,7 +225,8 @@ struct eimp_engine {
     void release_pools() {
         DevBuf *bufs[] = {&d_tg, &d_gt, &d_nerr, &d_bh, &d_bhflags, &d_mono, &d_splitsites,
                           &d_splits, &d_cMdiffs, &d_stats, &d_cref, &d_count0, &d_out,
-                          &d_digest, &d_tlist, &d_citems, &d_pitems, &d_tmp};
+                          &d_digest, &d_tlist, &d_citems, &d_pitems, &d_tmp,
+                          &p_blocks, &p_rows, &p_c0, &p_abs};
         for (auto *b : bufs) b->release();
         h_stage.release();
     }
@@ -981,6 +986,42 @@ int eimp_engine_copy_target(eimp_engine *e, size_t t, uint32_t *besthaps, uint64
     CHK(set_device(e));
     return copy_target_out(e->h_tg[t], e->d_tg.as<TgtDev>(), (uint32_t)t, e->Nad, e->stream, e->d_tmp,
                            besthaps, bhflags, splitsites, splits, cMdiffs, count0, pbwt, cref_rows);
+}
+
+int eimp_engine_ref_pbwt(eimp_engine *e, size_t m0, size_t nsites, uint32_t *rows, int32_t *count0,
+                         int32_t *absperm, double *device_ms) {
+    if (!e || !e->initialized) return fail(EIMP_ERR_STATE, "engine not initialized");
+    if (!rows || !count0) return fail(EIMP_ERR_INVALID, "null rows/count0");
+    if (nsites < 1 || m0 >= e->M || nsites > e->M - m0)
+        return fail(EIMP_ERR_INVALID, "site range outside the panel");
+    if (nsites >= (1u << 31)) return fail(EIMP_ERR_UNSUPPORTED, "too many sites");
+    const uint64_t N = e->Nad, Kpad = round_up(N, 64);
+    if (Kpad > panel_pbwt_max_k())
+        return fail(EIMP_ERR_UNSUPPORTED, "panel wider than the PBWT workgroup (" +
+                                              std::to_string(panel_pbwt_max_k()) + " haplotypes)");
+    CHK(set_device(e));
+    const uint64_t nblk = (nsites + 15) / 16, RS = 2 * ((N + 31) / 32);
+    CHK(e->p_blocks.ensure(nblk * Kpad * 2, "panel PBWT blocks"));
+    CHK(e->p_rows.ensure(nsites * RS * 4, "panel PBWT rows"));
+    CHK(e->p_c0.ensure(nsites * 4, "panel PBWT count0"));
+    if (absperm) CHK(e->p_abs.ensure((nsites + 1) * N * 4, "panel PBWT permutations"));
+    hipEvent_t a = e->get_event(), b = e->get_event();
+    if (a) (void)hipEventRecord(a, e->stream);
+    HIPCHK(launch_panel_pbwt(e->pd, m0, (uint32_t)nsites, e->p_blocks.as<uint16_t>(), e->p_rows.as<uint32_t>(),
+                             e->p_c0.as<int32_t>(), absperm ? e->p_abs.as<int32_t>() : nullptr, e->stream));
+    if (b) (void)hipEventRecord(b, e->stream);
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (device_ms) {
+        float ms = 0;
+        if (a && b) (void)hipEventElapsedTime(&ms, a, b);
+        *device_ms = ms;
+    }
+    if (a) e->event_pool.push_back(a);
+    if (b) e->event_pool.push_back(b);
+    HIPCHK(hipMemcpy(rows, e->p_rows.p, nsites * RS * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(count0, e->p_c0.p, nsites * 4, hipMemcpyDeviceToHost));
+    if (absperm) HIPCHK(hipMemcpy(absperm, e->p_abs.p, (nsites + 1) * N * 4, hipMemcpyDeviceToHost));
+    return EIMP_OK;
 }
 
 int eimp_engine_set_timing(eimp_engine *e, int enable) {

@@ -862,8 +862,12 @@ __device__ __forceinline__ uint32_t lane_scan_excl(uint32_t v) {
 // NS = slots per wave for this target.  Wave w owns chunks [w*NS, w*NS+NS);
 // chunks >= Kpad/64 are padding (all-ones bits, never moving), so every slot
 // is processed unconditionally.
-template <int NT, int NS>
-__device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint32_t *smem) {
+// ABS (panel PBWT for imputation): also writes the order after every step to
+// absperm (row step+1; row 0 = identity) and the step's count0 to c0out.
+template <int NT, int NS, bool ABS = false>
+__device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint32_t *smem,
+                                           int32_t *__restrict__ absperm = nullptr,
+                                           int32_t *__restrict__ c0out = nullptr) {
     constexpr int NW = NT / 64;
     constexpr uint32_t NPOS = NS * NW * 64;
     static_assert(NS <= 64, "one output lane per slot");
@@ -890,6 +894,15 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
             // the cref kernel writes all-ones for K <= k < Kpad
             const uint32_t v = pos < Kpad ? (uint32_t)cref[(uint64_t)b0 * Kpad + pos] : 0xFFFFu;
             p[c] = (pos << 16) | v;
+        }
+    }
+    if constexpr (ABS) {
+        if (absperm) {
+#pragma unroll
+            for (int c = 0; c < NS; c++) {
+                const uint32_t pos = (ch0 + (uint32_t)c) * 64 + ln;
+                if (pos < K) absperm[pos] = (int32_t)pos;
+            }
         }
     }
     uint32_t *__restrict__ outbase = T.out + (rev ? (uint64_t)Mp * RS : 0ull);
@@ -966,6 +979,17 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < NS; c++) p[c] = pay[(ch0 + (uint32_t)c) * 64 + ln];
+        if constexpr (ABS) {
+            if (absperm) {
+                int32_t *__restrict__ ap = absperm + (uint64_t)(step + 1) * K;
+#pragma unroll
+                for (int c = 0; c < NS; c++) {
+                    const uint32_t pos = (ch0 + (uint32_t)c) * 64 + ln;
+                    if (pos < K) ap[pos] = (int32_t)(p[c] >> 16);
+                }
+            }
+            if (threadIdx.x == 0) c0out[step] = (int32_t)cnt0;
+        }
         // the row: chunk ch holds pairs 2ch and 2ch+1 (raw layout: words 2ch, 2ch+1)
         uint32_t *__restrict__ row = outbase + (uint64_t)step * RS;
         if (ostore) {
@@ -1038,6 +1062,58 @@ __global__ __launch_bounds__(NT) void k_pbwt(const TgtDev *__restrict__ tg,
     extern __shared__ uint32_t smem[];
     const PbwtItem it = items[blockIdx.x];
     pbwt_target<NT, CPW>(tg[it.t], it.dir != 0, smem);
+}
+
+// ---------------------------------------------------------------------------
+// Panel PBWT for imputation (PBWT(refT, true), src/PBWT.cpp:53-71,131-153):
+// one workgroup, forward only, K = all panel haplotypes, the absolute order
+// after every site and count0 per site written out.
+// ---------------------------------------------------------------------------
+template <int NT, int CPW>
+__global__ __launch_bounds__(NT) void k_panel_pbwt(TgtDev T, int32_t *__restrict__ absperm,
+                                                  int32_t *__restrict__ c0out) {
+    extern __shared__ uint32_t smem[];
+    const int ns = pbwt_slots_for(T.Kpad, NT);
+    static_for<0, kNumSlotCounts>([&](auto ii) {
+        constexpr int NSv = kSlotCounts[decltype(ii)::value];
+        if constexpr (NSv <= CPW) {
+            if (ns == NSv) pbwt_steps<NT, NSv, true>(T, false, smem, absperm, c0out);
+        }
+    });
+}
+
+// Site rows [m0, m0 + nsites) of the panel as 16-row blocks, one uint16 per
+// haplotype (bit j = site m0 + 16b + j), haplotypes N..Kpad-1 all ones.
+// grid (nblk, ceil(nw/256)), block 256: a thread transposes one 64-haplotype word.
+__global__ __launch_bounds__(256) void k_site_blocks(PanelDev P, uint64_t m0, uint32_t nsites,
+                                                    uint16_t *__restrict__ blocks, uint32_t Kpad) {
+    const uint32_t b = blockIdx.x, q = blockIdx.y * 256 + threadIdx.x;
+    const uint32_t nw = (P.N_ad + 63) / 64;
+    if (q >= nw) return;
+    uint32_t V[32];  // two uint16 values per register: haplotypes 2i, 2i+1 of the low word etc.
+#pragma unroll
+    for (int i = 0; i < 32; i++) V[i] = 0;
+    for (uint32_t r = 0; r < 16; r++) {
+        const uint32_t j = 16 * b + r;
+        const uint64_t wv = j < nsites ? P.site[(m0 + j) * P.site_stride + q] : 0ull;
+        const uint32_t lo = (uint32_t)wv, hi = (uint32_t)(wv >> 32);
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            V[i] |= (((lo >> (2 * i)) & 1u) << r) | (((lo >> (2 * i + 1)) & 1u) << (16 + r));
+            V[16 + i] |= (((hi >> (2 * i)) & 1u) << r) | (((hi >> (2 * i + 1)) & 1u) << (16 + r));
+        }
+    }
+    if (64 * q + 64 > P.N_ad) {
+#pragma unroll
+        for (int i = 0; i < 32; i++) {
+            const uint32_t k0 = 64 * q + 2 * i;
+            if (k0 >= P.N_ad) V[i] |= 0xFFFFu;
+            if (k0 + 1 >= P.N_ad) V[i] |= 0xFFFF0000u;
+        }
+    }
+    uint4 *d4 = (uint4 *)(blocks + (uint64_t)b * Kpad + 64 * q);
+#pragma unroll
+    for (int i = 0; i < 8; i++) d4[i] = make_uint4(V[4 * i], V[4 * i + 1], V[4 * i + 2], V[4 * i + 3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1466,6 +1542,30 @@ hipError_t launch_pbwt(int v, const TgtDev *tg, const PbwtItem *items, uint32_t 
                        items);
     return hipGetLastError();
 }
+
+// ---- panel PBWT for imputation: one workgroup of the 1024-thread variant
+hipError_t launch_panel_pbwt(const PanelDev &P, uint64_t m0, uint32_t nsites, uint16_t *blocks,
+                             uint32_t *rows, int32_t *c0out, int32_t *absperm, hipStream_t s) {
+    constexpr int NT = 1024, CPW = 24;
+    TgtDev T{};
+    T.K = P.N_ad;
+    T.Kpad = (P.N_ad + 63) / 64 * 64;
+    T.Mp = nsites;
+    T.RS = row_stride(T.K, LAYOUT_COMPACT);
+    T.cref = blocks;
+    T.out = rows;
+    const int ns = pbwt_slots_for(T.Kpad, NT);
+    if (ns < 0 || ns > CPW) return hipErrorInvalidValue;
+    const uint32_t nw = (P.N_ad + 63) / 64;
+    hipLaunchKernelGGL(k_site_blocks, dim3((nsites + 15) / 16, (nw + 255) / 256), dim3(256), 0, s, P,
+                       m0, nsites, blocks, T.Kpad);
+    const size_t lds = (size_t)ns * NT * 4 + (NT / 64) * 4;
+    hipError_t e = raise_lds_limit((const void *)k_panel_pbwt<NT, CPW>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_panel_pbwt<NT, CPW>), dim3(1), dim3(NT), lds, s, T, absperm, c0out);
+    return hipGetLastError();
+}
+uint32_t panel_pbwt_max_k() { return 1024 / 64 * 24 * 64; }
 
 // ---- device-driven batch (stream contexts)
 

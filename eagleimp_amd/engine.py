@@ -33,6 +33,7 @@ EXPORTS = [
     "eimp_pbwt_variant_count", "eimp_pbwt_variant_info", "eimp_stream_create",
     "eimp_stream_submit", "eimp_stream_query", "eimp_stream_wait", "eimp_stream_copy_target",
     "eimp_stream_destroy", "eimp_engine_mem_info", "eimp_stream_kernel_ms",
+    "eimp_engine_ref_pbwt",
 ]
 
 _P = C.c_void_p
@@ -85,6 +86,7 @@ def lib():
         L.eimp_stream_destroy.argtypes = [_P]
         L.eimp_engine_mem_info.argtypes = [_P, C.POINTER(_SZ), C.POINTER(_SZ)]
         L.eimp_stream_kernel_ms.argtypes = [_P, _P]
+        L.eimp_engine_ref_pbwt.argtypes = [_P, _SZ, _SZ, _P, _P, _P, _P]
         _LIB = L
     return _LIB
 
@@ -232,6 +234,18 @@ class Engine:
         S, Mp, _, K = (int(x) for x in self._last[t, :4])
         return _copy_out(lib().eimp_engine_copy_target, self._h, t, S, Mp, K, self._last_W[t],
                          self._last_layout, self.Nad, cref)
+
+    def ref_pbwt(self, m0=0, nsites=None, absperm=True):
+        """Panel PBWT for imputation over sites [m0, m0+nsites): (rows, count0,
+        absperm or None, device ms)."""
+        nsites = self.M - m0 if nsites is None else nsites
+        N = self.Nad
+        rows = np.zeros((nsites, 2 * ((N + 31) // 32)), np.uint32)
+        c0 = np.zeros(nsites, np.int32)
+        ap = np.zeros((nsites + 1, N), np.int32) if absperm else None
+        ms = np.zeros(1, np.float64)
+        _check(lib().eimp_engine_ref_pbwt(self._h, m0, nsites, _p(rows), _p(c0), _p(ap), _p(ms)))
+        return rows, c0, ap, float(ms[0])
 
     def mem_info(self):
         """(free, total) device bytes."""

@@ -153,6 +153,17 @@ int eimp_engine_copy_target(eimp_engine *engine, size_t t, uint32_t *besthaps, u
                             uint32_t *splitsites, uint64_t *splits, float *cMdiffs,
                             int32_t *count0, uint32_t *pbwt, uint64_t *cref_rows);
 
+/* Panel PBWT for imputation: the reference's PBWT(refT, true)
+ * (src/PBWT.cpp:53-71,131-153,207-285, built by src/PBWTImputer.cpp:110,119)
+ * over sites [m0, m0+nsites) of the resident panel with all NrefhapsAD
+ * haplotypes, forward only: rows receives nsites rows of 2*ceil(N/32) uint32
+ * (data / cumulative zero pairs, 1-padding), count0 nsites int32, and absperm,
+ * if not NULL, (nsites+1) x N int32 absolute orders (row m = order before site
+ * m0+m; getSortOrder, src/PBWT.h:78).  device_ms (NULL or 1 double): device
+ * time.  N up to 24,576 (one workgroup carries the order). */
+int eimp_engine_ref_pbwt(eimp_engine *engine, size_t m0, size_t nsites, uint32_t *rows,
+                         int32_t *count0, int32_t *absperm, double *device_ms);
+
 /* ---------------------------------------------------------------------- */
 /* Stream contexts: asynchronous, device-driven batches.  No counterpart in  */
 /* the reference, whose GPU calls are synchronous on the default stream     */

@@ -75,6 +75,8 @@ def lib():
         L.eo_mix64.restype = _U64
         L.eo_mix64.argtypes = [_U64]
         L.eo_digest.argtypes = [_P, _U64, _U64, _U64, _P, _P, _P]
+        L.eo_build_ref_pbwt.argtypes = [_P, _U64, _U64, _U64, _P, _P, _P]
+        L.eo_build_ref_pbwt.restype = None
         L.eo_process_target.restype = C.c_int
         L.eo_process_target.argtypes = [_P, _P, C.c_int, _P]
         L.eo_result_free.argtypes = [_P]
@@ -307,4 +309,16 @@ def ref_build_ref_pbwt(rows, K, absperm=True):
     ap = np.zeros((M + 1, K), np.int32) if absperm else None
     if ref().ref_build_ref_pbwt(_p(rows), rows.shape[1], K, M, _p(out), _p(c0), _p(ap)) != 0:
         raise RuntimeError("ref_build_ref_pbwt failed")
+    return out, c0, ap
+
+
+def build_ref_pbwt(rows, K, absperm=True):
+    """Panel PBWT for imputation over site rows (uint64, bit h = haplotype h):
+    (rows of 2*ceil(K/32) uint32, count0, absolute permutations (M+1) x K)."""
+    rows = np.ascontiguousarray(rows, np.uint64)
+    M = rows.shape[0]
+    out = np.zeros((M, 2 * ((K + 31) // 32)), np.uint32)
+    c0 = np.zeros(M, np.int32)
+    ap = np.zeros((M + 1, K), np.int32) if absperm else None
+    lib().eo_build_ref_pbwt(_p(rows), rows.shape[1], K, M, _p(out), _p(c0), _p(ap))
     return out, c0, ap

@@ -379,6 +379,56 @@ void eo_build_pbwt(const uint64_t *cref, uint64_t cref_stride, uint64_t K, uint6
 }
 
 /* ---------------------------------------------------------------------- */
+/* Panel PBWT for imputation: PBWT(refT, true), src/PBWT.cpp:53-71,131-153 */
+/* and buildComprPBWT (:207-285) with refPBWT set: forward only over M site */
+/* rows of K haplotypes, count0 computed per site (haveCount0 false,        */
+/* :228,275-276), the absolute permutation before every site kept (:218-222, */
+/* absperm row m; row M = after the last site).                            */
+/* ---------------------------------------------------------------------- */
+void eo_build_ref_pbwt(const uint64_t *rows, uint64_t row_stride, uint64_t K, uint64_t M,
+                       uint32_t *out, int32_t *count0, int32_t *absperm) {
+    uint64_t KW = eo_pbwt_row_words(K);
+    int32_t *perm = (int32_t *)malloc((K ? K : 1) * sizeof(int32_t));
+    int32_t *nperm = (int32_t *)malloc((K ? K : 1) * sizeof(int32_t));
+    for (uint64_t i = 0; i < K; i++) perm[i] = (int32_t)i;
+    if (absperm) memcpy(absperm, perm, K * sizeof(int32_t));
+    for (uint64_t m = 0; m < M; m++) {
+        const uint64_t *x = rows + m * row_stride;
+        uint64_t c0 = 0;
+        for (uint64_t i = 0; i < K; i++) c0 += !bit_at(x, i);
+        uint32_t *cp = out + m * KW;
+        uint64_t p0 = 0, p1 = c0;
+        uint32_t data = 0, mask = 1;
+        for (uint64_t i = 0; i < K; i++) {
+            int32_t e = perm[i];
+            if (!bit_at(x, (uint64_t)e)) {
+                nperm[p0++] = e;
+            } else {
+                data |= mask;
+                nperm[p1++] = e;
+            }
+            mask <<= 1;
+            if (mask == 0) {
+                *cp++ = data;
+                *cp++ = (uint32_t)p0;
+                data = 0;
+                mask = 1;
+            }
+        }
+        if (K % 32) {
+            mask = ~(mask - 1);
+            *cp++ = data | mask;
+            *cp++ = (uint32_t)p0;
+        }
+        count0[m] = (int32_t)p0;
+        int32_t *tmp = perm; perm = nperm; nperm = tmp;
+        if (absperm) memcpy(absperm + (m + 1) * K, perm, K * sizeof(int32_t));
+    }
+    free(perm);
+    free(nperm);
+}
+
+/* ---------------------------------------------------------------------- */
 /* a10: drop-in GPU layout (src/GPUHandler.cpp:127,259; src/GPUEngine.cu:160,182-211) */
 /* ---------------------------------------------------------------------- */
 void eo_pbwt_to_gpu_layout(const uint32_t *cpu, uint64_t K, uint64_t Mp, uint32_t *gpu) {

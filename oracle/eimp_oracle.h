@@ -95,6 +95,13 @@ void eo_condensed_ref(const eo_panel *p, const eo_target *t,
 
 /* a9: src/PBWT.cpp:33-51,131-153,207-285.  cref rows of cref_stride words,
  * out has 2*Mp rows of eo_pbwt_row_words(K) uint32 (forward then backward). */
+/* Panel PBWT for imputation (PBWT(refT, true), src/PBWT.cpp:53-71,131-153):
+ * forward only over M site rows (row_stride uint64 words, bit h = haplotype h)
+ * of K haplotypes; out M rows of eo_pbwt_row_words(K) uint32, count0 M,
+ * absperm (M+1) x K (row m = order before site m) or NULL. */
+void eo_build_ref_pbwt(const uint64_t *rows, uint64_t row_stride, uint64_t K, uint64_t M,
+                       uint32_t *out, int32_t *count0, int32_t *absperm);
+
 void eo_build_pbwt(const uint64_t *cref, uint64_t cref_stride, uint64_t K,
                    uint64_t Mp, const int32_t *count0, uint32_t *out);
 

@@ -11,6 +11,9 @@ it writes are plain data (inputs and the reference's outputs) in .npz files:
   ref_genotype.npz  genotype codes -> the reference GenotypeVector words
                     (src/Datatypes.cpp:27-56) and compare_withPreInit error
                     counts (src/Datatypes.cpp:107-116)
+  ref_panel_pbwt.npz  site rows -> the reference's panel PBWT for imputation,
+                    PBWT(refT, true): forward rows, count0 and the absolute
+                    permutations (src/PBWT.cpp:53-71,131-153,207-285)
 
 Usage:  python tests/golden/make_golden.py
 """
@@ -95,11 +98,32 @@ def genotype_fixtures():
     np.savez_compressed(os.path.join(HERE, "ref_genotype.npz"), **out)
 
 
+PANEL_CASES = [  # (K, M, density, seed)
+    (1, 3, 0.5, 21), (31, 20, 0.3, 22), (64, 33, 0.5, 23), (77, 60, 0.5, 24), (300, 17, 0.1, 25),
+]
+
+
+def panel_pbwt_fixtures():
+    out = {}
+    for i, (K, M, dens, seed) in enumerate(PANEL_CASES):
+        rng = np.random.default_rng(seed)
+        rows, _ = random_cref(K, M, dens, rng)
+        pbwt, count0, absperm = oracle.ref_build_ref_pbwt(rows, K)
+        out[f"p{i}_K"] = np.int64(K)
+        out[f"p{i}_rows"] = rows
+        out[f"p{i}_pbwt"] = pbwt
+        out[f"p{i}_count0"] = count0
+        out[f"p{i}_absperm"] = absperm
+    out["ncases"] = np.int64(len(PANEL_CASES))
+    np.savez_compressed(os.path.join(HERE, "ref_panel_pbwt.npz"), **out)
+
+
 if __name__ == "__main__":
     oracle.build()
     if not oracle.ref_available():
         sys.exit("reference harness not built (is /root/reference present?)")
     pbwt_fixtures()
     genotype_fixtures()
-    for f in ("ref_pbwt.npz", "ref_genotype.npz"):
+    panel_pbwt_fixtures()
+    for f in ("ref_pbwt.npz", "ref_genotype.npz", "ref_panel_pbwt.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")

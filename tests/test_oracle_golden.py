@@ -124,3 +124,24 @@ def test_raw_layout_ingested_by_reference(built):
         nw = (K + 31) // 32
         assert np.array_equal(got[:, :2 * nw], rows), (K, Mp)
         assert np.array_equal(gc0, c0), (K, Mp)
+
+
+def test_panel_pbwt_golden(built):
+    """The oracle's panel PBWT for imputation against the reference's
+    PBWT(refT, true) outputs: rows, count0 and every absolute permutation."""
+    g = np.load(os.path.join(GOLD, "ref_panel_pbwt.npz"))
+    for i in range(int(g["ncases"])):
+        K = int(g[f"p{i}_K"])
+        rows, c0, ap = oracle.build_ref_pbwt(g[f"p{i}_rows"], K)
+        assert np.array_equal(rows, g[f"p{i}_pbwt"]), (i, K)
+        assert np.array_equal(c0, g[f"p{i}_count0"]), (i, K)
+        assert np.array_equal(ap, g[f"p{i}_absperm"]), (i, K)
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference harness not built here")
+def test_panel_pbwt_live_against_reference(built):
+    p = synth.make_panel(5, N=333, M=700, F=10, T=2)
+    a = oracle.build_ref_pbwt(p.site, p.N)
+    b = oracle.ref_build_ref_pbwt(p.site, p.N)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
