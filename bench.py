@@ -282,6 +282,7 @@ def main():
 
     graph_tps = None
     tuned = None
+    nctx = None
     if args.engine == "batch":
         # synchronous engine: one run_batch call per step
         for w in range(args.warmup):
@@ -487,7 +488,7 @@ def main():
             "workload": cfg["name"], "N": panel.N, "M": panel.M, "targets": T,
             "targets_per_step": batch, "K": sorted(set(int(k) for k in K_all)),
             "window": "full" if cfg.get("wmin") is None else "log-uniform",
-            "mean_Mp": round(float(Mp_all.mean()), 1),
+            "mean_Mp": round(float(Mp_all.mean()), 1), "contexts": nctx,
         },
         "bit_exact": bit_exact,
         "b_alg_GBps": round(balg / wall / 1e9, 2),

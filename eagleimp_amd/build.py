@@ -46,6 +46,11 @@ def build_synth():
           "-o", SYNTH, os.path.join(CSRC, "synth.c"), "-lm"])
 
 
+# host-only phasing code: float results must equal the reference's, so no
+# contraction into fused multiply-adds
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-ffp-contract=off", "-pthread"]
+
+
 def build_engine():
     hip = _hipcc()
     objs = []
@@ -54,8 +59,11 @@ def build_engine():
         _run([hip, f"--offload-arch={ARCH}", *HIP_FLAGS, "-x", "hip", "-c",
               os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    obj = os.path.join(CSRC, "_eimp_phase.o")
+    _run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, "eimp_phase.cpp"), "-o", obj])
+    objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([hip, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([hip, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)

@@ -614,6 +614,11 @@ int stream_setup(eimp_stream *s, size_t pool_bytes) {
 
 }  // namespace
 
+namespace eimp {
+// error message for the C ABI entry points of other translation units
+void set_last_error(const std::string &msg) { g_err = msg; }
+}  // namespace eimp
+
 extern "C" {
 
 const char *eimp_last_error(void) { return g_err.c_str(); }

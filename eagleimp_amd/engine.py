@@ -33,7 +33,7 @@ EXPORTS = [
     "eimp_pbwt_variant_count", "eimp_pbwt_variant_info", "eimp_stream_create",
     "eimp_stream_submit", "eimp_stream_query", "eimp_stream_wait", "eimp_stream_copy_target",
     "eimp_stream_destroy", "eimp_engine_mem_info", "eimp_stream_kernel_ms",
-    "eimp_engine_ref_pbwt",
+    "eimp_engine_ref_pbwt", "eimp_phase_default_params", "eimp_phase_target", "eimp_phase_batch",
 ]
 
 _P = C.c_void_p

@@ -88,9 +88,15 @@ def tune(eng: Engine, targets, idx, batches=(128, 256), reserve_bytes: int = 24 
     free, _ = eng.mem_info()
     pool = max(free - reserve_bytes, 1 << 28)
     table = []
+    cap = len(idx)  # targets one context's pool holds (learnt from deferrals)
     for v in vs:
         eng.set_pbwt_variant(v["variant"])
-        for b in batches:
+        todo, seen = list(batches), set()
+        while todo:
+            b = min(todo.pop(0), cap)
+            if b in seen:
+                continue
+            seen.add(b)
             st = eng.stream(b, Wmax, Kmax, pool, timed=True, graph=False)
             sub = idx[:b]
             t0 = time.perf_counter()
@@ -108,14 +114,19 @@ def tune(eng: Engine, targets, idx, batches=(128, 256), reserve_bytes: int = 24 
                        ms_per_target=dev / max(done, 1), pbwt_ms=ms["pbwt"], cref_ms=ms["cref"],
                        wall_s=round(wall, 3),
                        bytes_per_target=float(region_bytes(res.Mp[fit], res.K[fit]).mean())
-                       if done else 0.0)
+                       if done else 0.0, deferred=b - done)
             table.append(row)
             if log:
                 log(f"tune: {row}")
+            if 0 < done < b:  # the pool holds fewer: measure at its capacity instead
+                cap = done
+                todo.insert(0, cap)
+    # submissions that deferred targets did their preparation twice: out
+    fitting = [r for r in table if r["deferred"] == 0] or table
     # near-ties (within 2%) go to the lowest variant index, so that repeated
     # runs pick the same kernel; then the fastest submission size of it
-    fastest = min(r["ms_per_target"] for r in table)
-    near = [r for r in table if r["ms_per_target"] <= 1.02 * fastest]
+    fastest = min(r["ms_per_target"] for r in fitting)
+    near = [r for r in fitting if r["ms_per_target"] <= 1.02 * fastest]
     v0 = min(r["variant"] for r in near)
     best = min((r for r in near if r["variant"] == v0), key=lambda r: r["ms_per_target"])
     choice = dict(variant=best["variant"], threads=best["threads"],

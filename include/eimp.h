@@ -132,7 +132,7 @@ typedef struct eimp_target_result {
     uint64_t Mp;         /* 2*S+1 PBWT rows per direction */
     uint64_t ncalls;     /* call (het split) sites */
     uint64_t K;          /* K after clamping */
-    uint64_t digest_pbwt;   /* sum of fmix32(v + g*0x9E3779B1) over logical words */
+    uint64_t digest_pbwt;   /* sum of rotl32(v ^ g*0x9E3779B1, g mod 32) over logical words */
     uint64_t digest_count0; /* sum of fmix32(c0 + r*0x7FEB352D + 0x68E31DA4) */
 } eimp_target_result;
 
@@ -253,6 +253,85 @@ int eimp_engine_reset_timing(eimp_engine *engine);
 int eimp_engine_set_pbwt_variant(eimp_engine *engine, int variant);
 int eimp_pbwt_variant_count(void);
 int eimp_pbwt_variant_info(int variant, int *threads, int *chunks_per_wave, int *max_k);
+
+/* ------------------------------------------------------------------------ */
+/* Beam-search phasing over a target's compressed PBWT (host code): the     */
+/* consumer of the engine's output.  The reference's GPU handler wraps each */
+/* target's cPBWT as PBWT(cpbwt, count0, K, 4*Kwords, Mp) and calls          */
+/* Target::phaseExtPBWT (src/GPUHandler.cpp:259-266, src/Target.cpp:291-777,*/
+/* src/Beam.cpp, src/BeamPath.cpp, src/HaplotypePath.cpp, src/PBWT.cpp:      */
+/* 288-382).  Results are identical to the reference's (same float          */
+/* arithmetic, same pruning order).                                         */
+/* ------------------------------------------------------------------------ */
+
+/* Parameters; eimp_phase_default_params gives the reference's defaults
+ * (src/Target.cpp:41-51; src/Args.cpp:276-278, src/main.cpp:305-306,326:
+ * hist 1, pErr 0.003, pLimit = pErr, expectIBDcM 2; reverse phasing on,
+ * pre-phasing off, last iteration, missing genotypes not imputed). */
+typedef struct eimp_phase_params {
+    uint32_t history;          /* HistorySize (100); times hist, at most 128 */
+    uint32_t history_fast;     /* HistorySizeFast (30), pre-phasing */
+    uint32_t beam_width;       /* BeamWidth (50) */
+    uint32_t beam_width_fast;  /* BeamWidthFast (30) */
+    uint32_t delta;            /* Delta (20) */
+    uint32_t delta_fast;       /* DeltaFast (10) */
+    float hist;                /* history length multiplier */
+    float p_err;               /* genotype error probability */
+    float p_limit;             /* beam probability deviation limit */
+    float expect_ibd_cM;       /* expected IBD length (cM) */
+    float min_fix_size, max_fix_size, min_fix_thresh, max_fix_thresh; /* 0.5 0.9 0.99 1.0 */
+    int32_t pre_phasing;       /* doPrePhasing */
+    int32_t rev_phasing;       /* doRevPhasing */
+    int32_t last_iter;         /* lastIter */
+    int32_t imp_missing;       /* impMissing (needs af) */
+} eimp_phase_params;
+
+void eimp_phase_default_params(eimp_phase_params *params);
+
+/* One target.  pbwt: 2*Mp rows of row_stride uint32 holding (data, cumulative
+ * zero count) pairs, forward rows first, then the backward rows in processing
+ * order (the drop-in layout has row_stride 4*Kwords, the compact CPU layout
+ * 2*ceil(K/32)).  gt: GenotypeVector words over the window's W sites.
+ * splitsites: the S = (Mp-1)/2 window-relative split sites, cMdiffs S+1
+ * floats (both as eimp_engine_copy_target returns them).  af: NULL or W
+ * allele frequencies (used with imp_missing).  start_phase: haplotype of the
+ * maternal path at the first call site, 0 Ref / 1 Alt
+ * (VCFData::getStartPhase). */
+typedef struct eimp_phase_input {
+    const uint32_t *pbwt;
+    const int32_t *count0;
+    uint32_t K, row_stride, Mp, W;
+    const uint64_t *gt;
+    const uint32_t *splitsites;
+    const float *cMdiffs;
+    const float *af;
+    int32_t start_phase;
+    int32_t pad;
+} eimp_phase_input;
+
+/* mat/pat: ceil(W/64) words each (bit m = allele at window site m),
+ * dosage_mat/pat: W floats each, call_conf: NULL or one float per call site
+ * (written in the last iteration).  total_conf, ncallsites, nmissing,
+ * nmonhets: the target's totals (Target::getTotalConfidence,
+ * getNCallSites, getMissings, getMonHets). */
+typedef struct eimp_phase_output {
+    uint64_t *mat, *pat;
+    float *dosage_mat, *dosage_pat;
+    float *call_conf;
+    float total_conf;
+    uint32_t ncalls;
+    uint64_t ncallsites;
+    int32_t nmissing, nmonhets;
+} eimp_phase_output;
+
+int eimp_phase_target(const eimp_phase_params *params, const eimp_phase_input *in,
+                      eimp_phase_output *out);
+
+/* n independent targets over `threads` host threads (<= 0: hardware
+ * concurrency), as the reference's phasing threads take targets from their
+ * queue (src/PBWTPhaser.cpp:490-499).  Returns the first failure. */
+int eimp_phase_batch(const eimp_phase_params *params, size_t n, const eimp_phase_input *in,
+                     eimp_phase_output *out, int threads);
 
 #ifdef __cplusplus
 }

@@ -104,6 +104,8 @@ def ref():
         R.ref_ingest_raw.argtypes = [_P, _U64, _U64, _U64, _P, _P]
         R.ref_build_ref_pbwt.restype = C.c_int
         R.ref_build_ref_pbwt.argtypes = [_P, _U64, _U64, _U64, _P, _P, _P]
+        R.ref_phase_target.restype = C.c_int
+        R.ref_phase_target.argtypes = [_P, _P, _P, _P]
         _REF = R
     return _REF
 
@@ -310,6 +312,18 @@ def ref_build_ref_pbwt(rows, K, absperm=True):
     if ref().ref_build_ref_pbwt(_p(rows), rows.shape[1], K, M, _p(out), _p(c0), _p(ap)) != 0:
         raise RuntimeError("ref_build_ref_pbwt failed")
     return out, c0, ap
+
+
+def ref_phase_target(params, job, monos):
+    """The reference's own beam search (src/Beam.cpp, src/BeamPath.cpp,
+    src/HaplotypePath.cpp) driven as Target::phaseExtPBWT over the inputs of a
+    phasing job (eagleimp_amd.phase.job); monos: the target's mono bits over
+    the window.  Returns a PhaseResult."""
+    monos = np.ascontiguousarray(monos, np.uint64)
+    if ref().ref_phase_target(C.byref(params), C.byref(job.inp), _p(monos),
+                              C.byref(job.out)) != 0:
+        raise RuntimeError("ref_phase_target failed")
+    return job.result()
 
 
 def build_ref_pbwt(rows, K, absperm=True):

@@ -1,7 +1,8 @@
-// ref_harness.cpp -- C entry points around the reference's own PBWT and
-// GenotypeVector code (src/PBWT.cpp, src/Datatypes.cpp, src/RingBuffer.cpp of
-// ikmb/eagleimp), compiled in place from /root/reference by oracle/Makefile
-// into oracle/_ref/libeimp_ref.so.
+// ref_harness.cpp -- C entry points around the reference's own PBWT,
+// GenotypeVector and beam search code (src/PBWT.cpp, src/Datatypes.cpp,
+// src/RingBuffer.cpp, src/Beam.cpp, src/BeamPath.cpp, src/HaplotypePath.cpp,
+// src/BooleanHistory.cpp of ikmb/eagleimp), compiled in place from
+// /root/reference by oracle/Makefile into oracle/_ref/libeimp_ref.so.
 //
 // TEST INFRASTRUCTURE ONLY: used to write golden vectors (tests/golden) and
 // to cross-check the oracle where the reference sources are present.
@@ -14,11 +15,17 @@
 #include <string>
 #include <vector>
 
+#include <cmath>
+#include <iostream>  // the reference headers expect it included first
+
+#include "Beam.h"
 #include "Datatypes.h"
+#include "MinMaxHeap.h"
 #include "MyMalloc.h"
 #include "PBWT.h"
 #include "RingBuffer.h"
 #include "StatusFile.h"
+#include "eimp.h"
 
 // static members normally defined by the reference's main.cpp (src/main.cpp:48-50)
 MyMalloc MyMalloc::mm;
@@ -146,6 +153,295 @@ int ref_build_ref_pbwt(const uint64_t *rows, uint64_t row_stride, uint64_t K, ui
             const auto &o = pbwt.getSortOrder(m);
             std::memcpy(absperm + m * K, o.data(), K * sizeof(int32_t));
         }
+    return 0;
+}
+
+// Target::phaseExtPBWT (src/Target.cpp:291-777) restated around the
+// reference's own Beam over PBWT(cpbwt, count0, K, row_stride, Mp)
+// (src/GPUHandler.cpp:259-260).  Target.cpp itself needs VCFData (HTSlib),
+// so its driver loop is written out here, statement for statement; monos:
+// the target's mono bits over the window (src/Target.cpp:866-897).
+int ref_phase_target(const eimp_phase_params *P, const eimp_phase_input *in,
+                     const uint64_t *monos, eimp_phase_output *out) {
+    struct Conf {  // src/Target.h:181-186
+        size_t constraint_idx;
+        fp_type p;
+        Conf(size_t i, fp_type p_) : constraint_idx(i), p(p_) {}
+        bool operator<(const Conf &o) const { return p < o.p; }
+    };
+    const size_t MFull = in->W, S = (in->Mp - 1) / 2, K = in->K;
+    auto geno = [&](size_t m) {
+        const uint64_t b = 1ull << (m & 63);
+        const bool is0 = in->gt[2 * (m >> 6)] & b, is2 = in->gt[2 * (m >> 6) + 1] & b;
+        return is0 ? (is2 ? Genotype::Miss : Genotype::HomRef)
+                   : (is2 ? Genotype::HomAlt : Genotype::Het);
+    };
+    auto mono = [&](size_t m) { return (monos[m >> 6] >> (m & 63)) & 1ull; };
+    std::vector<size_t> csplits(in->splitsites, in->splitsites + S);
+    std::vector<fp_type> cMsplitDiffs(in->cMdiffs, in->cMdiffs + (S ? S + 1 : 0));
+    std::vector<size_t> callLocs;
+    GenotypeVector target;
+    for (size_t i = 0; i < S; i++) {
+        target.push_back(geno(csplits[i]));
+        if (geno(csplits[i]) == Genotype::Het) callLocs.push_back(i);
+    }
+    // the PBWT object owns (and frees) its buffers
+    size_t bytes = 2 * in->Mp * (size_t)in->row_stride * sizeof(uint32_t);
+    uint32_t *cp = (uint32_t *)MyMalloc::malloc(bytes, "phase_cpbwt");
+    std::memcpy(cp, in->pbwt, bytes);
+    auto *c0 = new std::vector<int>(in->count0, in->count0 + in->Mp);
+    PBWT pbwt(cp, c0, K, in->row_stride, in->Mp, g_next_id++);
+
+    std::vector<float> dmat(MFull, 0.0f), dpat(MFull, 0.0f);
+    std::vector<bool> pmat, ppat;
+    fp_type totalconf = 0;
+    size_t ncallsites = 0;
+    int totalmissings = 0, totalmonhets = 0;
+    const bool doPrePhasing = P->pre_phasing, doRevPhasing = P->rev_phasing;
+    const bool lastIter = P->last_iter, impMissing = P->imp_missing;
+    const fp_type hist = P->hist, pErr = P->p_err, pLimit = P->p_limit;
+    const fp_type expectIBDcM = P->expect_ibd_cM;
+    const fp_type minFixSize = P->min_fix_size, maxFixSize = P->max_fix_size;
+    const fp_type minFixThresh = P->min_fix_thresh, maxFixThresh = P->max_fix_thresh;
+    const size_t HistorySize = P->history, HistorySizeFast = P->history_fast;
+
+    std::vector<Constraint> constraints(csplits.size(), Constraint(Constraint::Type::NoConstr, 0));
+    Haplotype currMatPhase = in->start_phase ? Haplotype::Alt : Haplotype::Ref;
+    Haplotype lastMatPhase = currMatPhase;
+
+    if (doPrePhasing) {
+        size_t history = static_cast<size_t>((HistorySizeFast * hist) + 0.5);
+        Beam beam(history, P->beam_width_fast, P->delta_fast, pErr, pLimit, expectIBDcM,
+                  cMsplitDiffs, callLocs, target, pbwt, constraints);
+        if (csplits.size() > 0) constraints[0] = Constraint(Constraint::Type::HetKeep, 0);
+        MinMaxHeap<Conf> callheap, homheap;
+        size_t next_call_idx = 1;
+        for (size_t si = 1; si < csplits.size(); si++) {
+            if (si == callLocs[next_call_idx]) {
+                fp_type p = beam.callKeepPhaseProb(next_call_idx) - 0.5;
+                Haplotype ref = currMatPhase;
+                Constraint c(Constraint::Type::NoConstr,
+                             callLocs[next_call_idx] - callLocs[next_call_idx - 1] - 1);
+                if (next_call_idx > 1) {
+                    fp_type p2 = beam.callKeepPhaseProb(next_call_idx, true) - 0.5;
+                    if (std::abs(p2) > std::abs(p)) {
+                        p = p2;
+                        ref = lastMatPhase;
+                        c.refidx = callLocs[next_call_idx] - callLocs[next_call_idx - 2] - 1;
+                    }
+                }
+                lastMatPhase = currMatPhase;
+                if (p < 0) {
+                    currMatPhase = !ref;
+                    c.type = Constraint::Type::HetSwitch;
+                } else {
+                    currMatPhase = ref;
+                    c.type = Constraint::Type::HetKeep;
+                }
+                constraints[si] = c;
+                p = std::abs(p);
+                callheap.emplace(si, p);
+                next_call_idx++;
+            } else {
+                fp_type p = beam.callDosage(si);
+                if (target[si] == Genotype::HomRef) {
+                    constraints[si] = Constraint(Constraint::Type::HomRef, 0);
+                    p = 2.0 - p;
+                } else
+                    constraints[si] = Constraint(Constraint::Type::HomAlt, 0);
+                homheap.emplace(si, p);
+            }
+        }
+        size_t sizedivdmin = callheap.getSize() * minFixSize;
+        size_t sizedivdmax = callheap.getSize() * (1.0 - maxFixSize);
+        size_t si;
+        for (si = 0; si < sizedivdmin; si++) {
+            Conf cc = callheap.popMin();
+            if (cc.p >= maxFixThresh - 0.5 || (cc.p >= minFixThresh - 0.5 && si >= sizedivdmax))
+                break;
+            constraints[cc.constraint_idx].type = Constraint::Type::NoConstr;
+        }
+        if (si == sizedivdmin) {
+            for (; si < callheap.getSize(); si++) {
+                Conf cc = callheap.popMin();
+                if (cc.p > 0.0) break;
+                constraints[cc.constraint_idx].type = Constraint::Type::NoConstr;
+            }
+        }
+        sizedivdmin = homheap.getSize() * minFixSize;
+        sizedivdmax = homheap.getSize() * (1.0 - maxFixSize);
+        for (si = 0; si < sizedivdmin; si++) {
+            Conf cc = homheap.popMin();
+            if (cc.p >= maxFixThresh * 2.0 || (cc.p >= minFixThresh * 2.0 && si >= sizedivdmax))
+                break;
+            constraints[cc.constraint_idx].type = Constraint::Type::NoConstr;
+        }
+    }
+
+    currMatPhase = in->start_phase ? Haplotype::Alt : Haplotype::Ref;
+    std::vector<fp_type> phaseProbs;
+    phaseProbs.push_back(0);
+    auto plain_site = [&](size_t m, Haplotype &mat, Haplotype &pat) {
+        switch (geno(m)) {
+        case Genotype::HomAlt:
+            mat = Haplotype::Alt;
+            pat = Haplotype::Alt;
+            dmat[m] = 1.0;
+            dpat[m] = 1.0;
+            break;
+        case Genotype::Miss:
+            if (impMissing) {
+                float af = in->af[m];
+                dmat[m] = af;
+                dpat[m] = af;
+                mat = af > 0.5 ? Haplotype::Alt : Haplotype::Ref;
+                pat = mat;
+            }
+            if (lastIter) totalmissings++;
+            break;
+        default:
+            break;
+        }
+    };
+    {
+        size_t history = static_cast<size_t>((HistorySize * hist) + 0.5);
+        Beam beam(history, P->beam_width, P->delta, pErr, pLimit, expectIBDcM, cMsplitDiffs,
+                  callLocs, target, pbwt, constraints);
+        size_t m = 0;
+        while (csplits.size() > 0 ? m < csplits[callLocs[0]] : m < MFull) {
+            Haplotype mat = Haplotype::Ref, pat = Haplotype::Ref;
+            if (geno(m) == Genotype::Het) {
+                mat = Haplotype::Ref;
+                pat = Haplotype::Alt;
+                dmat[m] = 0.5;
+                dpat[m] = 0.5;
+            } else
+                plain_site(m, mat, pat);
+            pmat.push_back(toBool(mat));
+            ppat.push_back(toBool(pat));
+            m++;
+        }
+        if (m < MFull) {
+            pmat.push_back(toBool(currMatPhase));
+            ppat.push_back(!toBool(currMatPhase));
+            m++;
+        }
+        size_t next_call_idx = 1;
+        for (; m < MFull; m++) {
+            Haplotype mat = Haplotype::Ref, pat = Haplotype::Ref;
+            if (geno(m) == Genotype::Het) {
+                if (!mono(m)) {
+                    fp_type p = beam.callKeepPhaseProb(next_call_idx) - 0.5;
+                    Haplotype ref = currMatPhase;
+                    phaseProbs.push_back(p);
+                    if (p < 0)
+                        currMatPhase = !ref;
+                    else
+                        currMatPhase = ref;
+                    next_call_idx++;
+                } else {
+                    dmat[m] = 0.5;
+                    dpat[m] = 0.5;
+                    if (lastIter) totalmonhets++;
+                }
+                mat = currMatPhase;
+                pat = !currMatPhase;
+            } else
+                plain_site(m, mat, pat);
+            pmat.push_back(toBool(mat));
+            ppat.push_back(toBool(pat));
+        }
+    }
+
+    std::vector<fp_type> callConfs;
+    if (lastIter && csplits.size() > 0) {
+        callConfs.resize(callLocs.size(), 0.0);
+        if (doRevPhasing) {
+            callConfs[callLocs.size() - 1] = std::abs(phaseProbs.back()) + 0.5;
+        } else {
+            auto cc = callConfs.begin();
+            for (const auto &p : phaseProbs) {
+                *cc = std::abs(p) + 0.5;
+                cc++;
+            }
+        }
+    }
+    if (lastIter && doRevPhasing && csplits.size() > 0) {
+        pbwt.switchToReverse();
+        size_t history = static_cast<size_t>((HistorySize * hist) + 0.5);
+        std::vector<Constraint> revconstraints(csplits.size(),
+                                               Constraint(Constraint::Type::NoConstr, 0));
+        for (size_t cidx = 1; cidx < csplits.size(); cidx++) {
+            Constraint &c = constraints[cidx];
+            switch (c.type) {
+            case Constraint::Type::HetKeep:
+            case Constraint::Type::HetSwitch: {
+                Constraint &rc = revconstraints[cidx - c.refidx - 1];
+                if (rc.type != Constraint::Type::NoConstr) {
+                    size_t ridx = cidx - c.refidx + rc.refidx;
+                    revconstraints[ridx].refidx = c.refidx - rc.refidx - 1;
+                    if (c.type != rc.type)
+                        revconstraints[ridx].type = Constraint::Type::HetSwitch;
+                    else
+                        revconstraints[ridx].type = Constraint::Type::HetKeep;
+                } else {
+                    rc.type = c.type;
+                    rc.refidx = c.refidx;
+                }
+            } break;
+            case Constraint::Type::HomRef:
+            case Constraint::Type::HomAlt:
+                revconstraints[cidx].type = c.type;
+                break;
+            default:
+                break;
+            }
+        }
+        revconstraints[csplits.size() - 1] = Constraint(Constraint::Type::HetKeep, 0);
+        float dref = phaseProbs.back() < 0.0 ? 0.5 + phaseProbs.back() : 0.5 - phaseProbs.back();
+        float dalt = phaseProbs.back() < 0.0 ? 0.5 - phaseProbs.back() : 0.5 + phaseProbs.back();
+        dmat[csplits[callLocs.back()]] = currMatPhase == Haplotype::Ref ? dref : dalt;
+        dpat[csplits[callLocs.back()]] = currMatPhase == Haplotype::Ref ? dalt : dref;
+        Beam beam(history, P->beam_width, P->delta, pErr, pLimit, expectIBDcM, cMsplitDiffs,
+                  callLocs, target, pbwt, revconstraints, true);
+        for (int rci = callLocs.size() - 2; rci >= 0; rci--) {
+            fp_type p = beam.callKeepPhaseProb(rci) - 0.5;
+            Haplotype ref = currMatPhase;
+            if (std::abs(phaseProbs[rci + 1]) > std::abs(p)) p = phaseProbs[rci + 1];
+            if (p < 0)
+                currMatPhase = !ref;
+            else
+                currMatPhase = ref;
+            pmat[csplits[callLocs[rci]]] = toBool(currMatPhase);
+            ppat[csplits[callLocs[rci]]] = !toBool(currMatPhase);
+            dref = p < 0.0 ? 0.5 + p : 0.5 - p;
+            dalt = p < 0.0 ? 0.5 - p : 0.5 + p;
+            dmat[csplits[callLocs[rci]]] = currMatPhase == Haplotype::Ref ? dref : dalt;
+            dpat[csplits[callLocs[rci]]] = currMatPhase == Haplotype::Ref ? dalt : dref;
+            callConfs[rci] = std::abs(p) + 0.5;
+        }
+    }
+    if (lastIter && csplits.size() > 0) {
+        for (size_t cci = 0; cci < callConfs.size(); cci++) totalconf += callConfs[cci];
+        ncallsites += callConfs.size();
+    }
+
+    const size_t words = (MFull + 63) / 64;
+    std::memset(out->mat, 0, words * 8);
+    std::memset(out->pat, 0, words * 8);
+    for (size_t m = 0; m < MFull; m++) {
+        if (pmat[m]) out->mat[m >> 6] |= 1ull << (m & 63);
+        if (ppat[m]) out->pat[m >> 6] |= 1ull << (m & 63);
+    }
+    std::memcpy(out->dosage_mat, dmat.data(), MFull * sizeof(float));
+    std::memcpy(out->dosage_pat, dpat.data(), MFull * sizeof(float));
+    if (out->call_conf && !callConfs.empty())
+        std::memcpy(out->call_conf, callConfs.data(), callConfs.size() * sizeof(float));
+    out->total_conf = totalconf;
+    out->ncalls = (uint32_t)callLocs.size();
+    out->ncallsites = ncallsites;
+    out->nmissing = totalmissings;
+    out->nmonhets = totalmonhets;
     return 0;
 }
 

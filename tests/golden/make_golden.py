@@ -14,6 +14,12 @@ it writes are plain data (inputs and the reference's outputs) in .npz files:
   ref_panel_pbwt.npz  site rows -> the reference's panel PBWT for imputation,
                     PBWT(refT, true): forward rows, count0 and the absolute
                     permutations (src/PBWT.cpp:53-71,131-153,207-285)
+  ref_phase.npz     the reference's beam search (src/Beam.cpp,
+                    src/BeamPath.cpp, src/HaplotypePath.cpp) driven as
+                    Target::phaseExtPBWT (src/Target.cpp:291-777) on the
+                    phasing cases of tests/phase_cases.py: phased haplotypes,
+                    dosages, call confidences and totals, with a hash of the
+                    inputs (regenerated from their seeds by the test)
 
 Usage:  python tests/golden/make_golden.py
 """
@@ -24,8 +30,11 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+sys.path.insert(0, os.path.dirname(HERE))
 
 import oracle  # noqa: E402
+import phase_cases  # noqa: E402
+from eagleimp_amd import phase  # noqa: E402
 
 PBWT_CASES = [  # (K, Mp, density, seed)
     (2, 1, 0.5, 1), (2, 9, 0.5, 2), (31, 7, 0.3, 3), (32, 33, 0.5, 4), (33, 33, 0.5, 5),
@@ -118,6 +127,32 @@ def panel_pbwt_fixtures():
     np.savez_compressed(os.path.join(HERE, "ref_panel_pbwt.npz"), **out)
 
 
+PHASE_GOLDEN = [("tiny_beam_small_K", 1), ("missing_imputed_no_reverse", 0),
+                ("roh_hom_splits_prephase", 0), ("long_roh_deep_history", 2),
+                ("few_founders_full_panel", 0), ("noisy_wide_beam", 1)]
+
+
+def phase_fixtures():
+    out = {}
+    for i, (name, seed) in enumerate(PHASE_GOLDEN):
+        c = phase_cases.make_case(name, seed)
+        j = phase_cases.job_for(c)
+        res = oracle.ref_phase_target(phase_cases.params_for(c), j, c["r"]["monos"])
+        out[f"c{i}_name"] = np.array(name)
+        out[f"c{i}_seed"] = np.int64(seed)
+        out[f"c{i}_inhash"] = np.array(phase_cases.input_hash(c))
+        out[f"c{i}_mat"] = np.packbits(res.mat, bitorder="little")
+        out[f"c{i}_pat"] = np.packbits(res.pat, bitorder="little")
+        out[f"c{i}_dm"] = res.dosage_mat
+        out[f"c{i}_dp"] = res.dosage_pat
+        out[f"c{i}_cc"] = res.call_conf
+        out[f"c{i}_tot"] = np.float32(res.total_conf)
+        out[f"c{i}_counts"] = np.array([res.ncalls, res.ncallsites, res.nmissing, res.nmonhets],
+                                       np.int64)
+    out["ncases"] = np.int64(len(PHASE_GOLDEN))
+    np.savez_compressed(os.path.join(HERE, "ref_phase.npz"), **out)
+
+
 if __name__ == "__main__":
     oracle.build()
     if not oracle.ref_available():
@@ -125,5 +160,6 @@ if __name__ == "__main__":
     pbwt_fixtures()
     genotype_fixtures()
     panel_pbwt_fixtures()
-    for f in ("ref_pbwt.npz", "ref_genotype.npz", "ref_panel_pbwt.npz"):
+    phase_fixtures()
+    for f in ("ref_pbwt.npz", "ref_genotype.npz", "ref_panel_pbwt.npz", "ref_phase.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")

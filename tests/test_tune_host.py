@@ -24,14 +24,17 @@ class _Stream:
 
     def wait(self):
         a = np.ones((self.n, 6), np.uint64)
-        return BatchResult(*(a[:, i] for i in range(6))), 0, (0.0, 0.0)
+        a[self.eng.cap:, 1] = 0  # Mp == 0: deferred, the pool is full
+        return BatchResult(*(a[:, i] for i in range(6))), max(0, self.n - self.eng.cap), (0.0, 0.0)
 
     def kernel_ms(self):
         v = self.eng.variant
-        # variant 3 is fastest per target, and larger batches amortise better
+        # variant 3 is fastest per target, and larger batches amortise better;
+        # preparation runs for every submitted target, the rest for those placed
         per = self.eng.per.get(v, 9.0)
-        return {"distance": 0.0, "select": 0.1, "monos": 0.0, "splits": 0.1,
-                "cref": 0.2 * self.n, "pbwt": per * self.n * (1.0 if self.batch > 128 else 1.2),
+        done = min(self.n, self.eng.cap)
+        return {"distance": 0.5 * self.n, "select": 0.1, "monos": 0.0, "splits": 0.1,
+                "cref": 0.2 * done, "pbwt": per * done * (1.0 if self.batch > 128 else 1.2),
                 "copy": 5.0}
 
     def close(self):
@@ -42,9 +45,10 @@ class _Engine:
     Nad = Ncorr = 20000
     M = 200000
 
-    def __init__(self, per=None):
+    def __init__(self, per=None, cap=10**9):
         self.variant = -1
         self.submits = 0
+        self.cap = cap  # targets a pool holds
         # device ms per target of each PBWT variant
         self.per = per or {2: 2.0, 3: 1.0, 4: 1.5}
 
@@ -84,3 +88,14 @@ def test_near_tie_goes_to_lowest_variant(tmp_path, monkeypatch):
     eng = _Engine(per={2: 1.005, 3: 1.0, 4: 1.5})  # 2 is within 2% of 3
     best = tune.tune(eng, _Targets(300, 10000), range(300), batches=(128, 256), use_cache=False)
     assert (best["variant"], best["batch"]) == (2, 256)
+
+
+def test_batches_capped_at_pool_capacity(tmp_path, monkeypatch):
+    """When a pool holds fewer targets than a candidate batch, the tuner
+    measures at the capacity and never picks a batch that defers targets."""
+    monkeypatch.setattr(tune, "CACHE", str(tmp_path / "tune.json"))
+    eng = _Engine(cap=150)
+    best = tune.tune(eng, _Targets(300, 10000), range(300), batches=(256, 300), use_cache=False)
+    assert best["batch"] == 150 and best["variant"] == 3
+    assert all(r["batch"] <= 150 or r["deferred"] > 0 for r in best["table"])
+    assert {r["batch"] for r in best["table"] if r["deferred"] == 0} == {150}
