@@ -59,9 +59,10 @@ def build_engine():
         _run([hip, f"--offload-arch={ARCH}", *HIP_FLAGS, "-x", "hip", "-c",
               os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
-    obj = os.path.join(CSRC, "_eimp_phase.o")
-    _run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, "eimp_phase.cpp"), "-o", obj])
-    objs.append(obj)
+    for src in ("eimp_phase.cpp", "eimp_qref.cpp"):
+        obj = os.path.join(CSRC, "_" + os.path.splitext(src)[0] + ".o")
+        _run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
     tmp = LIB + ".tmp"
     _run([hip, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, LIB)

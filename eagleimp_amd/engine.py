@@ -34,6 +34,8 @@ EXPORTS = [
     "eimp_stream_submit", "eimp_stream_query", "eimp_stream_wait", "eimp_stream_copy_target",
     "eimp_stream_destroy", "eimp_engine_mem_info", "eimp_stream_kernel_ms",
     "eimp_engine_ref_pbwt", "eimp_phase_default_params", "eimp_phase_target", "eimp_phase_batch",
+    "eimp_qref_open", "eimp_qref_close", "eimp_qref_metadata", "eimp_qref_variant",
+    "eimp_qref_read_sites",
 ]
 
 _P = C.c_void_p

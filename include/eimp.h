@@ -333,6 +333,48 @@ int eimp_phase_target(const eimp_phase_params *params, const eimp_phase_input *i
 int eimp_phase_batch(const eimp_phase_params *params, size_t n, const eimp_phase_input *in,
                      eimp_phase_output *out, int threads);
 
+/* ------------------------------------------------------------------------ */
+/* Qref reference panel reader (host code): loads the reference's binary     */
+/* panel format (qref_spec.md; VCFData::qRefOpenReadMeta and                 */
+/* qRefLoadNextVariant, src/VCFData.cpp:2107-2432; run-length code           */
+/* src/utils.h:133-315) into the site-major rows eimp_engine_initialize      */
+/* takes.  Sample masking and target matching (VCF/HTSlib) are not part of   */
+/* it.                                                                       */
+/* ------------------------------------------------------------------------ */
+typedef struct eimp_qref eimp_qref;
+
+typedef struct eimp_qref_info {
+    int32_t v_major, v_minor;   /* format version (1.<=1 accepted, src/VCFData.h:413-414) */
+    int32_t chrom;              /* 1..24, 23 = X */
+    int32_t has_haploid_flags;  /* per-sample flags stored in the file */
+    uint64_t Nref;              /* samples (haplotypes: 2*Nref, haploids encoded homozygous) */
+    uint64_t NrefHap;           /* haploid samples */
+    uint64_t Mref;              /* variants */
+    uint64_t MrefMA;            /* variants split from multi-allelics */
+    uint64_t hap_words;         /* uint64 words per stored haplotype vector (2*Nref rounded to 512 bits) */
+} eimp_qref_info;
+
+/* Maps the file, checks header and sizes, indexes the variants. */
+int eimp_qref_open(const char *path, eimp_qref **qref, eimp_qref_info *info);
+int eimp_qref_close(eimp_qref *qref);
+
+/* Any pointer may be NULL.  positions: Mref int64 (0-based), af: Mref float,
+ * haploid: Nref bytes (1 = haploid; all equal when the file stores no
+ * flags), ma_flags: ceil(Mref/64) words (bit m = split from a multi-allelic). */
+int eimp_qref_metadata(eimp_qref *qref, int64_t *positions, float *af, uint8_t *haploid,
+                       uint64_t *ma_flags);
+
+/* Alleles (reference/major first) and ID of variant m: zero-terminated
+ * strings valid until eimp_qref_close. */
+int eimp_qref_variant(eimp_qref *qref, size_t m, const char **ref, const char **alt,
+                      const char **id);
+
+/* Haplotypes of variants [m0, m0+n) as n site-major rows of row_words uint64
+ * (bit h = allele of haplotype h; row_words >= ceil(2*Nref/64), bits past
+ * 2*Nref zero), decoded over `threads` host threads (<= 0: all). */
+int eimp_qref_read_sites(eimp_qref *qref, size_t m0, size_t n, uint64_t *rows, size_t row_words,
+                         int threads);
+
 #ifdef __cplusplus
 }
 #endif

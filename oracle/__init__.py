@@ -106,6 +106,10 @@ def ref():
         R.ref_build_ref_pbwt.argtypes = [_P, _U64, _U64, _U64, _P, _P, _P]
         R.ref_phase_target.restype = C.c_int
         R.ref_phase_target.argtypes = [_P, _P, _P, _P]
+        R.ref_rle_encode.restype = C.c_int64
+        R.ref_rle_encode.argtypes = [_P, _U64, _P, _U64]
+        R.ref_rle_decode.restype = C.c_int
+        R.ref_rle_decode.argtypes = [_P, _U64, _P, _U64]
         _REF = R
     return _REF
 
@@ -324,6 +328,24 @@ def ref_phase_target(params, job, monos):
                               C.byref(job.out)) != 0:
         raise RuntimeError("ref_phase_target failed")
     return job.result()
+
+
+def ref_rle_encode(words):
+    """The reference's Qref run-length coder: bytes, or None when it gives up."""
+    words = np.ascontiguousarray(words, np.uint64)
+    out = np.zeros(words.size * 8 + 16, np.uint8)
+    n = ref().ref_rle_encode(_p(words), words.size, _p(out), out.size)
+    if n == -2:
+        raise RuntimeError("ref_rle_encode: buffer too small")
+    return None if n < 0 else out[:n].tobytes()
+
+
+def ref_rle_decode(code: bytes, nwords: int):
+    """The reference's Qref run-length decoder (no sample masking)."""
+    buf = np.frombuffer(code, np.uint8).copy() if code else np.zeros(1, np.uint8)
+    out = np.zeros(nwords, np.uint64)
+    ref().ref_rle_decode(_p(buf), len(code), _p(out), nwords)
+    return out
 
 
 def build_ref_pbwt(rows, K, absperm=True):

@@ -26,6 +26,7 @@
 #include "RingBuffer.h"
 #include "StatusFile.h"
 #include "eimp.h"
+#include "utils.h"
 
 // static members normally defined by the reference's main.cpp (src/main.cpp:48-50)
 MyMalloc MyMalloc::mm;
@@ -153,6 +154,29 @@ int ref_build_ref_pbwt(const uint64_t *rows, uint64_t row_stride, uint64_t K, ui
             const auto &o = pbwt.getSortOrder(m);
             std::memcpy(absperm + m * K, o.data(), K * sizeof(int32_t));
         }
+    return 0;
+}
+
+// The Qref run-length coder (src/utils.h:133-190): codes nwords uint64 of
+// haplotype bits; returns the code length, or -1 when the coder gives up
+// (the code would not be shorter), -2 when cap is too small.
+int64_t ref_rle_encode(const uint64_t *words, uint64_t nwords, uint8_t *out, uint64_t cap) {
+    std::vector<char> enc;
+    if (!runlengthEncode(words, nwords, enc)) return -1;
+    if (enc.size() > cap) return -2;
+    std::memcpy(out, enc.data(), enc.size());
+    return (int64_t)enc.size();
+}
+
+// Its decoder (src/utils.h:193-315) without sample masking: nwords uint64
+// (zeroed here first).
+int ref_rle_decode(const uint8_t *code, uint64_t n, uint64_t *words, uint64_t nwords) {
+    std::vector<char> enc(code, code + n);
+    std::memset(words, 0, nwords * sizeof(uint64_t));
+    std::vector<size_t> masked;
+    std::vector<bool> masked_haploid;
+    size_t masked_alleles = 0;
+    runlengthDecode(enc, words, nwords, masked, masked_haploid, masked_alleles);
     return 0;
 }
 

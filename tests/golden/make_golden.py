@@ -14,6 +14,9 @@ it writes are plain data (inputs and the reference's outputs) in .npz files:
   ref_panel_pbwt.npz  site rows -> the reference's panel PBWT for imputation,
                     PBWT(refT, true): forward rows, count0 and the absolute
                     permutations (src/PBWT.cpp:53-71,131-153,207-285)
+  ref_rle.npz       haplotype bit vectors -> the reference's Qref run-length
+                    code (src/utils.h:133-190) or its refusal (code not
+                    shorter than the raw vector)
   ref_phase.npz     the reference's beam search (src/Beam.cpp,
                     src/BeamPath.cpp, src/HaplotypePath.cpp) driven as
                     Target::phaseExtPBWT (src/Target.cpp:291-777) on the
@@ -153,6 +156,51 @@ def phase_fixtures():
     np.savez_compressed(os.path.join(HERE, "ref_phase.npz"), **out)
 
 
+def rle_vectors(rng):
+    """Haplotype vectors with sparse, dense, long-run and leading-one
+    structure, including runs longer than 32767 and 65534 bits."""
+    out = []
+    for t in range(48):
+        nw = int(rng.choice([8, 16, 64, 256]))
+        n = nw * 64
+        kind = t % 6
+        if kind == 0:
+            bits = rng.random(n) < rng.random() * 0.05
+        elif kind == 1:
+            bits = rng.random(n) < 0.5
+        elif kind == 2:
+            bits = np.zeros(n, bool)
+            a = int(rng.integers(0, n))
+            bits[a:a + int(rng.integers(1, n))] = True
+        elif kind == 3:
+            bits = np.ones(n, bool)
+            bits[int(rng.integers(0, n))] = False
+        elif kind == 4:
+            bits = np.repeat(rng.random(n // 128) < 0.5, 128)
+        else:
+            bits = rng.random(n) < 0.01
+            bits[0] = True
+        out.append(bits)
+    for n, ones in ((1100 * 64, True), (1100 * 64, False), (520 * 64, True)):
+        bits = np.full(n, ones)
+        bits[n // 3] = not ones
+        out.append(bits)
+    return [np.packbits(b.astype(np.uint8), bitorder="little").view(np.uint64) for b in out]
+
+
+def rle_fixtures():
+    rng = np.random.default_rng(77)
+    out = {}
+    vecs = rle_vectors(rng)
+    for i, w in enumerate(vecs):
+        code = oracle.ref_rle_encode(w)
+        out[f"v{i}_words"] = w
+        out[f"v{i}_code"] = np.frombuffer(code, np.uint8) if code is not None else np.zeros(0, np.uint8)
+        out[f"v{i}_coded"] = np.int64(code is not None)
+    out["nvec"] = np.int64(len(vecs))
+    np.savez_compressed(os.path.join(HERE, "ref_rle.npz"), **out)
+
+
 if __name__ == "__main__":
     oracle.build()
     if not oracle.ref_available():
@@ -161,5 +209,7 @@ if __name__ == "__main__":
     genotype_fixtures()
     panel_pbwt_fixtures()
     phase_fixtures()
-    for f in ("ref_pbwt.npz", "ref_genotype.npz", "ref_panel_pbwt.npz", "ref_phase.npz"):
+    rle_fixtures()
+    for f in ("ref_pbwt.npz", "ref_genotype.npz", "ref_panel_pbwt.npz", "ref_phase.npz",
+              "ref_rle.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")
