@@ -188,6 +188,81 @@ def panel_pbwt_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def phase_bench(args):
+    """The consumer of the engine's output: beam-search phasing
+    (eimp_phase_batch, host threads) of targets whose cPBWTs the device built
+    (drop-in layout, copied out as the reference's GPU handler receives them),
+    beside the reference's own beam search (oracle/_ref, one thread) on a
+    sample of the same targets, compared output for output."""
+    import os
+    from eagleimp_amd import phase, synth
+    from eagleimp_amd.engine import Engine
+    import oracle
+
+    n = args.targets or 32
+    cfg, panel, tg = synth.make_workload(args.workload, T=n)
+    eng = Engine(0)
+    eng.initialize(2, panel.M, panel.N, panel.N, panel.site)
+    eng.set_panel_extras(panel.cM, hap=panel.hap)
+    t = time.perf_counter()
+    res = eng.run_batch(tg.gt, tg.s0, tg.W, tg.K)
+    dev_s = time.perf_counter() - t
+    t = time.perf_counter()
+    jobs = [phase.engine_job(eng, i, tg.gt[i], tg.W[i]) for i in range(n)]
+    copy_s = time.perf_counter() - t
+    log(f"phasing: {n} targets of {cfg['name']}, device cPBWTs in {dev_s:.2f} s, "
+        f"copied in {copy_s:.2f} s (mean S {float(np.mean(res.S)):.0f})")
+    P = phase.default_params()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "16"))
+    t = time.perf_counter()
+    one = phase.phase_batch(P, jobs[:2], threads=1)
+    single_s = (time.perf_counter() - t) / 2
+    t = time.perf_counter()
+    got = phase.phase_batch(P, jobs, threads=threads)
+    wall = time.perf_counter() - t
+    log(f"phasing {n} targets on {threads} threads: {wall:.2f} s; one thread {single_s:.3f} s/target")
+    cpu = None
+    exact = None
+    if oracle.ref_available():
+        k = int(max(1, min(n, args.cpu_seconds / max(2 * single_s, 1e-3))))
+        exact = all(a.same(b) for a, b in zip(one, got[:2]))
+        t = time.perf_counter()
+        for i in range(k):
+            W = int(tg.W[i])
+            g = np.asarray(tg.gt[i], np.uint64)
+            is0 = np.unpackbits(np.ascontiguousarray(g[0::2]).view(np.uint8), bitorder="little")[:W]
+            is2 = np.unpackbits(np.ascontiguousarray(g[1::2]).view(np.uint8), bitorder="little")[:W]
+            het = (is0 == 0) & (is2 == 0)
+            het[jobs[i].splitsites] = False  # the beam reads the mono flags at hets only
+            monos = np.packbits(np.concatenate([het, np.zeros((-W) % 64, bool)]).astype(np.uint8),
+                                bitorder="little").view(np.uint64)
+            ref = oracle.ref_phase_target(P, phase.job(jobs[i].pbwt, jobs[i].count0, jobs[i].inp.K,
+                                                       tg.gt[i], W, jobs[i].splitsites,
+                                                       jobs[i].cMdiffs), monos)
+            exact = exact and ref.same(got[i])
+        ref_s = (time.perf_counter() - t) / k
+        log(f"reference beam: {k} targets, {ref_s:.3f} s/target; identical={exact}")
+        cpu = {"value": round(1.0 / ref_s, 3), "unit": "targets/s", "cores": 1,
+               "kind": "reference",
+               "sample": f"{k} targets: the reference's Beam/BeamPath/HaplotypePath (oracle/_ref) "
+                         f"driven as Target::phaseExtPBWT on the same device cPBWTs, one thread"}
+    out = {
+        "metric": "phasing: targets/s (beam search over device-built cPBWTs, host threads)",
+        "value": round(n / wall, 3), "unit": "targets/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(wall * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": cfg["name"] + "-phasing", "targets": n, "threads": threads,
+                   "mean_S": round(float(np.mean(res.S)), 1), "K": sorted(set(int(k) for k in res.K))},
+        "bit_exact": exact,
+        "single_thread_targets_per_s": round(1.0 / single_s, 3),
+        "device_cpbwt_targets_per_s": round(n / dev_s, 3),
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,6 +292,9 @@ def main():
                          "the autotuner on a sample of the workload first")
     ap.add_argument("--graph-steps", type=int, default=2,
                     help="steps replayed through captured hipGraphs after the timed region")
+    ap.add_argument("--phase", action="store_true",
+                    help="phasing (beam search) over device-built cPBWTs on host threads, "
+                         "beside the reference's beam")
     ap.add_argument("--panel-pbwt", action="store_true",
                     help="measure the panel PBWT for imputation on the workload's panel instead")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -251,6 +329,9 @@ def main():
         return
     if args.panel_pbwt:
         panel_pbwt_bench(args)
+        return
+    if args.phase:
+        phase_bench(args)
         return
 
     from eagleimp_amd import synth
