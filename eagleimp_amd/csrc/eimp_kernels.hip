@@ -1056,9 +1056,19 @@ __device__ __forceinline__ void pbwt_target(const TgtDev &T, const bool rev, uin
     });
 }
 
+// Waves per SIMD that one workgroup's LDS allows at the variant's capacity
+// (at most 6, above which the larger variants would spill registers): the
+// register budget is capped so that VGPRs do not lower the occupancy (512 x 20:
+// 80 VGPRs, three workgroups per CU instead of two at 86).
+constexpr int pbwt_waves_hint(int NT, int CPW) {
+    const int lds = CPW * NT * 4 + (NT / 64) * 4;
+    const int w = (163840 / lds) * (NT / 256);
+    return w < 1 ? 1 : (w > 6 ? 6 : w);
+}
+
 template <int NT, int CPW>
-__global__ __launch_bounds__(NT) void k_pbwt(const TgtDev *__restrict__ tg,
-                                            const PbwtItem *__restrict__ items) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(pbwt_waves_hint(NT, CPW))))
+void k_pbwt(const TgtDev *__restrict__ tg, const PbwtItem *__restrict__ items) {
     extern __shared__ uint32_t smem[];
     const PbwtItem it = items[blockIdx.x];
     pbwt_target<NT, CPW>(tg[it.t], it.dir != 0, smem);
@@ -1358,7 +1368,8 @@ __global__ __launch_bounds__(CREF_NT, 4) void k_cref_persist_ident(PanelDev P,
 
 // PBWT items from the queue
 template <int NT, int CPW>
-__global__ __launch_bounds__(NT) void k_pbwt_persist(const TgtDev *__restrict__ tg,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(pbwt_waves_hint(NT, CPW))))
+void k_pbwt_persist(const TgtDev *__restrict__ tg,
                                                     BatchDev *__restrict__ B,
                                                     const PbwtItem *__restrict__ items) {
     extern __shared__ uint32_t smem[];
