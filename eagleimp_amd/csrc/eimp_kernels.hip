@@ -864,7 +864,13 @@ __device__ __forceinline__ uint32_t lane_scan_excl(uint32_t v) {
 // is processed unconditionally.
 // ABS (panel PBWT for imputation): also writes the order after every step to
 // absperm (row step+1; row 0 = identity) and the step's count0 to c0out.
-template <int NT, int NS, bool ABS = false>
+// WS: how each wave sums the NW wave zero counts after the first barrier --
+// 0: NW LDS broadcasts read into scalars (short latency; best when the
+// workgroups sharing a CU are at different points, as with targets of mixed
+// K and windows), 1: one LDS read per lane and a DPP lane scan (fewer
+// instructions and scalar registers; best for uniform targets).  The
+// autotuner picks between the variants built with each.
+template <int NT, int NS, bool ABS = false, int WS = 0>
 __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint32_t *smem,
                                            int32_t *__restrict__ absperm = nullptr,
                                            int32_t *__restrict__ c0out = nullptr) {
@@ -951,11 +957,19 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
         __syncthreads();
         // zeros before my first chunk, and in the whole row (count0)
         uint32_t run = 0, cnt0 = 0;
+        if constexpr (WS == 0) {
 #pragma unroll
-        for (int w2 = 0; w2 < NW; w2++) {
-            const uint32_t v = __builtin_amdgcn_readfirstlane(wzero[w2]);
-            cnt0 += v;
-            run += (uint32_t)w2 < wave ? v : 0u;
+            for (int w2 = 0; w2 < NW; w2++) {
+                const uint32_t v = __builtin_amdgcn_readfirstlane(wzero[w2]);
+                cnt0 += v;
+                run += (uint32_t)w2 < wave ? v : 0u;
+            }
+        } else {
+            // lane w reads wave w's count (NW is a power of two)
+            const uint32_t wv = wzero[ln & (NW - 1)];
+            const uint32_t wex = lane_scan_excl<NW>(wv);
+            run = (uint32_t)__builtin_amdgcn_readlane((int)wex, (int)wave);
+            cnt0 = (uint32_t)__builtin_amdgcn_readlane((int)(wex + wv), NW - 1);
         }
         // lane c: zeros before chunk ch0 + c (lanes >= NS: don't care)
         const uint32_t zbv = run + lane_scan_excl<NS>(64u - (uint32_t)__popc(olo) - (uint32_t)__popc(ohi));
@@ -1045,13 +1059,13 @@ __host__ __device__ inline int pbwt_slots_for(uint32_t Kpad, int nt) {
     return -1;
 }
 
-template <int NT, int CPW>
+template <int NT, int CPW, int WS>
 __device__ __forceinline__ void pbwt_target(const TgtDev &T, const bool rev, uint32_t *smem) {
     const int ns = pbwt_slots_for(T.Kpad, NT);
     static_for<0, kNumSlotCounts>([&](auto ii) {
         constexpr int NSv = kSlotCounts[decltype(ii)::value];
         if constexpr (NSv <= CPW) {
-            if (ns == NSv) pbwt_steps<NT, NSv>(T, rev, smem);
+            if (ns == NSv) pbwt_steps<NT, NSv, false, WS>(T, rev, smem);
         }
     });
 }
@@ -1066,12 +1080,12 @@ constexpr int pbwt_waves_hint(int NT, int CPW) {
     return w < 1 ? 1 : (w > 6 ? 6 : w);
 }
 
-template <int NT, int CPW>
+template <int NT, int CPW, int WS>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(pbwt_waves_hint(NT, CPW))))
 void k_pbwt(const TgtDev *__restrict__ tg, const PbwtItem *__restrict__ items) {
     extern __shared__ uint32_t smem[];
     const PbwtItem it = items[blockIdx.x];
-    pbwt_target<NT, CPW>(tg[it.t], it.dir != 0, smem);
+    pbwt_target<NT, CPW, WS>(tg[it.t], it.dir != 0, smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -1087,7 +1101,7 @@ __global__ __launch_bounds__(NT) void k_panel_pbwt(TgtDev T, int32_t *__restrict
     static_for<0, kNumSlotCounts>([&](auto ii) {
         constexpr int NSv = kSlotCounts[decltype(ii)::value];
         if constexpr (NSv <= CPW) {
-            if (ns == NSv) pbwt_steps<NT, NSv, true>(T, false, smem, absperm, c0out);
+            if (ns == NSv) pbwt_steps<NT, NSv, true, 1>(T, false, smem, absperm, c0out);
         }
     });
 }
@@ -1367,7 +1381,7 @@ __global__ __launch_bounds__(CREF_NT, 4) void k_cref_persist_ident(PanelDev P,
 }
 
 // PBWT items from the queue
-template <int NT, int CPW>
+template <int NT, int CPW, int WS>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(pbwt_waves_hint(NT, CPW))))
 void k_pbwt_persist(const TgtDev *__restrict__ tg,
                                                     BatchDev *__restrict__ B,
@@ -1382,7 +1396,7 @@ void k_pbwt_persist(const TgtDev *__restrict__ tg,
         __syncthreads();
         if (i >= np) break;
         const PbwtItem it = items[i];
-        pbwt_target<NT, CPW>(tg[it.t], it.dir != 0, smem);
+        pbwt_target<NT, CPW, WS>(tg[it.t], it.dir != 0, smem);
         __syncthreads();
     }
 }
@@ -1522,10 +1536,12 @@ struct PbwtVariant {
     void (*fn)(const TgtDev *, const PbwtItem *);
     void (*fn_persist)(const TgtDev *, BatchDev *, const PbwtItem *);
 };
-#define PBWT_VARIANT(NT, CPW) {NT, CPW, k_pbwt<NT, CPW>, k_pbwt_persist<NT, CPW>}
+#define PBWT_VARIANT(NT, CPW, WS) {NT, CPW, k_pbwt<NT, CPW, WS>, k_pbwt_persist<NT, CPW, WS>}
 static const PbwtVariant kPbwtVariants[] = {
-    PBWT_VARIANT(256, 8),  PBWT_VARIANT(256, 16),  PBWT_VARIANT(512, 20),
-    PBWT_VARIANT(512, 32), PBWT_VARIANT(1024, 24),
+    PBWT_VARIANT(256, 8, 0),  PBWT_VARIANT(256, 16, 0),  PBWT_VARIANT(512, 20, 0),
+    PBWT_VARIANT(512, 32, 0), PBWT_VARIANT(1024, 24, 0),
+    // the same shapes with the DPP wave-count scan (uniform targets)
+    PBWT_VARIANT(512, 20, 1), PBWT_VARIANT(1024, 24, 1),
 };
 #undef PBWT_VARIANT
 constexpr int kNumPbwtVariants = sizeof(kPbwtVariants) / sizeof(kPbwtVariants[0]);

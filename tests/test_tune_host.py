@@ -67,7 +67,8 @@ def test_candidates_hold_k():
     for K in (100, 2000, 10000, 16384):
         for v in tune.candidates(K):
             assert v["max_k"] >= (K + 63) // 64 * 64
-    assert [v["variant"] for v in tune.candidates(10000)] == [2, 3, 4]
+    # 512 x 20, 512 x 32, 1024 x 24, then 512 x 20 and 1024 x 24 with the DPP wave scan
+    assert [v["variant"] for v in tune.candidates(10000)] == [2, 3, 4, 5, 6]
 
 
 def test_tune_picks_fastest_and_caches(tmp_path, monkeypatch):
@@ -77,7 +78,8 @@ def test_tune_picks_fastest_and_caches(tmp_path, monkeypatch):
     best = tune.tune(eng, tg, range(300), batches=(128, 256))
     assert (best["variant"], best["batch"]) == (3, 256) and not best["cached"]
     assert eng.variant == 3
-    assert len(best["table"]) == 3 * 2 and eng.submits == 6
+    nv = len(tune.candidates(10000))
+    assert len(best["table"]) == nv * 2 and eng.submits == nv * 2
     eng2 = _Engine()
     again = tune.tune(eng2, tg, range(300), batches=(128, 256))
     assert again["cached"] and again["variant"] == 3 and eng2.submits == 0 and eng2.variant == 3
