@@ -214,6 +214,13 @@ struct HapPath {
         clear_ext();
         ext_iv[0].clear();
         ext_iv[1].clear();
+        reserve();
+    }
+    // interval lists hold at most H entries: reserved once per pooled object
+    void reserve() {
+        iv.reserve(H + 1);
+        ext_iv[0].reserve(H + 1);
+        ext_iv[1].reserve(H + 1);
     }
     void clear_ext() {
         ext_p[0] = ext_p[1] = 0.0f;
@@ -229,10 +236,11 @@ struct HapPath {
         p = o.ext_p[e];
         sexp = o.ext_sexp;
         ph = o.ph;
-        iv = o.ext_iv[e];
+        iv.assign(o.ext_iv[e].begin(), o.ext_iv[e].end());
         clear_ext();
         ext_iv[0].clear();
         ext_iv[1].clear();
+        reserve();
         hist.push(e == 1);
         ph.push(o.ext_p[e]);
     }
@@ -279,7 +287,7 @@ struct HapPath {
     // probabilities of the 0- and 1-extensions at split site `cursor`; the
     // previously extended path `prev` shares its newest common sites' intervals
     int32_t extend(const std::vector<fp> &prec, size_t cursor, const HapPath *prev,
-                   const Cpbwt &pb, std::vector<fp> &tmp) {
+                   const Cpbwt &pb) {
         rescale();
         size_t ncopy = 0;
         if (prev) ncopy = std::min(H, 1 + hist.common(prev->hist));
@@ -289,31 +297,36 @@ struct HapPath {
         const bool calc0 = want[0], calc1 = want[1];
         fp p0 = 0.0f, p1 = 0.0f;
         const size_t depth = std::min(H, iv.size() + 1);
-        for (size_t i = 0; i < depth; i++) tmp[i] = ph.at(i) * prec[i];
+        // weight of history depth i: its path probability times the
+        // recombination probability (computed only for the depths visited)
+        auto weight = [&](size_t i) -> fp { return ph.at(i) * prec[i]; };
 
         // depth 0: every conditioning haplotype, split by its allele here
         const int c0 = pb.count0(2 * cursor + 1);
+        const fp w0 = weight(0);
         if (c0 > 0) {
-            if (calc0) p0 = (fp)c0 * tmp[0];
+            if (calc0) p0 = (fp)c0 * w0;
             o0.push_back(Iv{0, c0 - 1});
         }
         if ((size_t)c0 < (size_t)pb.K) {
-            if (calc1) p1 = (fp)((size_t)pb.K - (size_t)c0) * tmp[0];
+            if (calc1) p1 = (fp)((size_t)pb.K - (size_t)c0) * w0;
             o1.push_back(Iv{c0, pb.K - 1});
         }
         bool stop = false;
         size_t i = 1;
         for (const size_t n = std::min(ncopy, iv.size() + 1); !stop && i < n; i++) {
             stop = true;
-            if (prev->ext_iv[0].size() > i) {
+            const bool h0 = prev->ext_iv[0].size() > i, h1 = prev->ext_iv[1].size() > i;
+            const fp wi = (h0 && calc0) || (h1 && calc1) ? weight(i) : 0.0f;
+            if (h0) {
                 const Iv &t = prev->ext_iv[0][i];
-                if (calc0) p0 += (fp)t.size() * tmp[i];
+                if (calc0) p0 += (fp)t.size() * wi;
                 o0.push_back(t);
                 stop = false;
             }
-            if (prev->ext_iv[1].size() > i) {
+            if (h1) {
                 const Iv &t = prev->ext_iv[1][i];
-                if (calc1) p1 += (fp)t.size() * tmp[i];
+                if (calc1) p1 += (fp)t.size() * wi;
                 o1.push_back(t);
                 stop = false;
             }
@@ -331,13 +344,15 @@ struct HapPath {
                 pb.map(2 * cursor + 1, con, t0, t1);
             }
             stop = true;
-            if (!t0.empty()) {
-                if (calc0) p0 += (fp)t0.size() * tmp[i];
+            const bool h0 = !t0.empty(), h1 = !t1.empty();
+            const fp wi = (h0 && calc0) || (h1 && calc1) ? weight(i) : 0.0f;
+            if (h0) {
+                if (calc0) p0 += (fp)t0.size() * wi;
                 o0.push_back(t0);
                 stop = false;
             }
-            if (!t1.empty()) {
-                if (calc1) p1 += (fp)t1.size() * tmp[i];
+            if (h1) {
+                if (calc1) p1 += (fp)t1.size() * wi;
                 o1.push_back(t1);
                 stop = false;
             }
@@ -551,7 +566,6 @@ public:
         : H_(H), P_(P), delta_(delta), perr_(perr), plimit_(plimit), ibd_(ibd), cmd_(cmd),
           calls_(calls), target_(target), cons_(constraints), pb_(pb), rev_(reverse) {
         prec_.assign(H, 0.0f);
-        tmp_.assign(H, 0.0f);
         cursor_ = rev_ ? (int)cons_.size() : -1;
         HapPath *m = take();
         m->start(H_, false);
@@ -683,7 +697,7 @@ private:
             int32_t minse = 0, maxse = INT32_MIN;
             for (HapPath *hp : haps_) {
                 if (!(hp->want[0] || hp->want[1])) continue;
-                const int32_t se = hp->extend(prec_, (size_t)cursor_, prev, pb_, tmp_);
+                const int32_t se = hp->extend(prec_, (size_t)cursor_, prev, pb_);
                 prev = hp;
                 minse = std::min(minse, se);
                 maxse = std::max(maxse, se);
@@ -893,7 +907,7 @@ private:
     std::vector<PairPath> beam_, next_, branches_;
     std::vector<HapPath *> haps_, old_, spare_;
     std::vector<std::pair<int, int>> map_;
-    std::vector<fp> prec_, tmp_;
+    std::vector<fp> prec_;
     MinMaxHeap<Cand> heap_;
     std::unordered_map<size_t, size_t> keys_;
     std::vector<uint8_t> alive_;
