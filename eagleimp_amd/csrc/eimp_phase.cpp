@@ -1200,25 +1200,53 @@ struct Phaser {
     }
 };
 
+// The FPGA handler's raw rows (data words only, bits past K one) turned into
+// (data, cumulative zero count) pairs as PBWT(cpbwt, K, Kwords, M) does, with
+// count0 taken from each forward row's count after its last logical word
+// (src/PBWT.cpp:72-110).
+struct PairRows {
+    std::vector<uint32_t> rows;
+    std::vector<int32_t> c0;
+};
+void raw_to_pairs(const eimp_phase_input &in, PairRows &out) {
+    const size_t nw = (in.K + 31) / 32, Mp = in.Mp;
+    out.rows.resize(2 * Mp * 2 * nw);
+    out.c0.resize(Mp);
+    for (size_t r = 0; r < 2 * Mp; r++) {
+        const uint32_t *src = in.pbwt + r * (size_t)in.row_stride;
+        uint32_t *dst = out.rows.data() + r * 2 * nw;
+        uint32_t zeros = 0;
+        for (size_t k = 0; k < nw; k++) {
+            zeros += 32u - (uint32_t)__builtin_popcount(src[k]);
+            dst[2 * k] = src[k];
+            dst[2 * k + 1] = zeros;
+        }
+        if (r < Mp) out.c0[r] = (int32_t)zeros;
+    }
+}
+
 void validate(const eimp_phase_params *P, const eimp_phase_input *in,
               const eimp_phase_output *out) {
     if (!P || !in || !out) raise(EIMP_ERR_INVALID, "phasing: null argument");
-    if (!in->pbwt || !in->count0 || !in->gt || !out->mat || !out->pat || !out->dosage_mat ||
-        !out->dosage_pat)
+    if (!in->pbwt || (!in->count0 && !in->raw_rows) || !in->gt || !out->mat || !out->pat ||
+        !out->dosage_mat || !out->dosage_pat)
         raise(EIMP_ERR_INVALID, "phasing: null buffer");
     if (in->K == 0 || in->Mp == 0 || in->Mp % 2 == 0)
         raise(EIMP_ERR_INVALID, "phasing: K > 0 and an odd Mp = 2*S+1 are required");
-    if ((uint64_t)in->row_stride < 2ull * ((in->K + 31) / 32))
-        raise(EIMP_ERR_INVALID, "phasing: row stride below 2*ceil(K/32)");
+    if (in->raw_rows ? (uint64_t)in->row_stride < (in->K + 31) / 32
+                     : (uint64_t)in->row_stride < 2ull * ((in->K + 31) / 32))
+        raise(EIMP_ERR_INVALID, in->raw_rows ? "phasing: raw row stride below ceil(K/32)"
+                                             : "phasing: row stride below 2*ceil(K/32)");
     const size_t S = (in->Mp - 1) / 2;
     if (S > 0 && (!in->splitsites || !in->cMdiffs))
         raise(EIMP_ERR_INVALID, "phasing: split sites and cM differences required");
     for (size_t i = 0; i < S; i++)
         if (in->splitsites[i] >= in->W || (i > 0 && in->splitsites[i] <= in->splitsites[i - 1]))
             raise(EIMP_ERR_INVALID, "phasing: split sites must ascend inside the window");
-    for (size_t m = 0; m < in->Mp; m++)
-        if (in->count0[m] < 0 || (uint32_t)in->count0[m] > in->K)
-            raise(EIMP_ERR_INVALID, "phasing: count0 outside 0..K");
+    if (!in->raw_rows)
+        for (size_t m = 0; m < in->Mp; m++)
+            if (in->count0[m] < 0 || (uint32_t)in->count0[m] > in->K)
+                raise(EIMP_ERR_INVALID, "phasing: count0 outside 0..K");
     if (P->imp_missing && !in->af) raise(EIMP_ERR_INVALID, "phasing: imp_missing needs af");
     if (P->beam_width == 0 || (P->pre_phasing && P->beam_width_fast == 0))
         raise(EIMP_ERR_INVALID, "phasing: beam width 0");
@@ -1235,8 +1263,20 @@ int phase_one(const eimp_phase_params *P, const eimp_phase_input *in, eimp_phase
               std::string &err) {
     try {
         validate(P, in, out);
-        Phaser ph(*P, *in, *out);
-        ph.run();
+        if (in->raw_rows) {
+            PairRows pr;
+            raw_to_pairs(*in, pr);
+            eimp_phase_input conv = *in;
+            conv.pbwt = pr.rows.data();
+            conv.count0 = pr.c0.data();
+            conv.row_stride = (uint32_t)(2 * ((in->K + 31) / 32));
+            conv.raw_rows = 0;
+            Phaser ph(*P, conv, *out);
+            ph.run();
+        } else {
+            Phaser ph(*P, *in, *out);
+            ph.run();
+        }
         return EIMP_OK;
     } catch (const Failure &f) {
         err = f.msg;

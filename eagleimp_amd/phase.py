@@ -35,7 +35,8 @@ class PhaseParams(C.Structure):
 class PhaseInput(C.Structure):
     _fields_ = [("pbwt", _P), ("count0", _P), ("K", C.c_uint32), ("row_stride", C.c_uint32),
                 ("Mp", C.c_uint32), ("W", C.c_uint32), ("gt", _P), ("splitsites", _P),
-                ("cMdiffs", _P), ("af", _P), ("start_phase", C.c_int32), ("pad", C.c_int32)]
+                ("cMdiffs", _P), ("af", _P), ("start_phase", C.c_int32),
+                ("raw_rows", C.c_int32)]
 
 
 class PhaseOutput(C.Structure):
@@ -95,11 +96,12 @@ class PhaseResult:
 class _Job:
     """One target's ctypes input and output with the arrays they point to."""
 
-    def __init__(self, pbwt, count0, K, gt, W, splitsites, cMdiffs, af=None, start_phase=0):
+    def __init__(self, pbwt, count0, K, gt, W, splitsites, cMdiffs, af=None, start_phase=0,
+                 raw=False):
         pbwt = np.ascontiguousarray(pbwt, dtype=np.uint32)
         if pbwt.ndim != 2 or pbwt.shape[0] % 2:
             raise ValueError("pbwt must be 2*Mp rows")
-        self.count0 = np.ascontiguousarray(count0, dtype=np.int32)
+        self.count0 = None if count0 is None else np.ascontiguousarray(count0, dtype=np.int32)
         self.gt = np.ascontiguousarray(gt, dtype=np.uint64)
         self.splitsites = np.ascontiguousarray(splitsites, dtype=np.uint32)
         self.cMdiffs = np.ascontiguousarray(cMdiffs, dtype=np.float32)
@@ -110,7 +112,7 @@ class _Job:
         ncalls_max = max(1, (Mp - 1) // 2)
         self.inp = PhaseInput(_p(pbwt), _p(self.count0), int(K), pbwt.shape[1], Mp, W,
                               _p(self.gt), _p(self.splitsites), _p(self.cMdiffs), _p(self.af),
-                              int(start_phase), 0)
+                              int(start_phase), 1 if raw else 0)
         words = (W + 63) // 64
         self.mat = np.zeros(max(words, 1), np.uint64)
         self.pat = np.zeros(max(words, 1), np.uint64)
@@ -130,13 +132,14 @@ class _Job:
                            int(o.ncallsites), int(o.nmissing), int(o.nmonhets))
 
 
-def job(pbwt, count0, K, gt, W, splitsites, cMdiffs, af=None, start_phase=0) -> _Job:
-    return _Job(pbwt, count0, K, gt, W, splitsites, cMdiffs, af, start_phase)
+def job(pbwt, count0, K, gt, W, splitsites, cMdiffs, af=None, start_phase=0, raw=False) -> _Job:
+    return _Job(pbwt, count0, K, gt, W, splitsites, cMdiffs, af, start_phase, raw)
 
 
 def phase_target(params: PhaseParams, pbwt, count0, K, gt, W, splitsites, cMdiffs, af=None,
-                 start_phase=0) -> PhaseResult:
-    j = _Job(pbwt, count0, K, gt, W, splitsites, cMdiffs, af, start_phase)
+                 start_phase=0, raw=False) -> PhaseResult:
+    """raw: pbwt rows hold data words only (the FPGA handler's raw layout)."""
+    j = _Job(pbwt, count0, K, gt, W, splitsites, cMdiffs, af, start_phase, raw)
     _check(_bind().eimp_phase_target(C.byref(params), C.byref(j.inp), C.byref(j.out)))
     return j.result()
 
@@ -156,5 +159,6 @@ def engine_job(src, t: int, gt, W, af=None, start_phase=0) -> _Job:
     outputs are copied from the device; the target's genotypes are the caller's)."""
     got = src.copy_target(t)
     K = int(got["besthaps"].shape[0])
+    raw = getattr(src, "layout", getattr(src, "_last_layout", "gpu")) == "raw"
     return _Job(got["pbwt"], got["count0"], K, gt, W, got["splitsites"], got["cMdiffs"], af,
-                start_phase)
+                start_phase, raw)

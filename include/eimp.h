@@ -291,7 +291,10 @@ void eimp_phase_default_params(eimp_phase_params *params);
 /* One target.  pbwt: 2*Mp rows of row_stride uint32 holding (data, cumulative
  * zero count) pairs, forward rows first, then the backward rows in processing
  * order (the drop-in layout has row_stride 4*Kwords, the compact CPU layout
- * 2*ceil(K/32)).  gt: GenotypeVector words over the window's W sites.
+ * 2*ceil(K/32)); with raw_rows set, rows of data words only (the FPGA
+ * handler's raw buffer, EIMP_BATCH_RAW, bits past K one), turned into pairs
+ * as PBWT(cpbwt, K, Kwords, M) does (src/PBWT.cpp:72-110, consumed by
+ * src/FPGAHandler.cpp:861-905), count0 then optional.  gt: GenotypeVector words over the window's W sites.
  * splitsites: the S = (Mp-1)/2 window-relative split sites, cMdiffs S+1
  * floats (both as eimp_engine_copy_target returns them).  af: NULL or W
  * allele frequencies (used with imp_missing).  start_phase: haplotype of the
@@ -306,7 +309,7 @@ typedef struct eimp_phase_input {
     const float *cMdiffs;
     const float *af;
     int32_t start_phase;
-    int32_t pad;
+    int32_t raw_rows;
 } eimp_phase_input;
 
 /* mat/pat: ceil(W/64) words each (bit m = allele at window site m),

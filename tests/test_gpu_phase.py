@@ -38,8 +38,10 @@ def test_phase_device_cpbwt(built, name, seed):
     eng.close()
 
 
-def test_phase_stream_outputs(built):
-    """Targets of one stream submission (device-driven path) phase as on the CPU."""
+@pytest.mark.parametrize("raw", [False, True])
+def test_phase_stream_outputs(built, raw):
+    """Targets of one stream submission (device-driven path; the raw layout
+    too, as the FPGA handler's consumer takes it) phase as on the CPU."""
     c0 = pc.make_case("default", 1)
     panel = c0["panel"]
     # three targets over the same panel: the case's own and two windows of it
@@ -50,7 +52,7 @@ def test_phase_stream_outputs(built):
     Ks = [200, 100, 64]
     say("phase over stream outputs", len(gts), "targets")
     eng = panel_engine(panel)
-    st = eng.stream(4, W, 200, 1 << 30)
+    st = eng.stream(4, W, 200, 1 << 30, raw=raw)
     st.submit(gts, s0, Ws, Ks)
     res, nd, _ = st.wait()
     assert nd == 0

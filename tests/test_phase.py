@@ -112,6 +112,23 @@ def test_drop_in_layout_and_batch():
         assert got.same(want)
 
 
+@pytest.mark.parametrize("name", ["default", "roh_hom_splits_prephase", "missing_imputed_no_reverse"])
+def test_raw_rows_as_the_fpga_handler_passes_them(name):
+    """Rows of data words only (bits past K one), turned into pairs the way
+    PBWT(cpbwt, K, Kwords, M) does, phase exactly as the pair rows."""
+    c = pc.make_case(name, 2)
+    r = c["r"]
+    K = int(r["K"])
+    raw = oracle.raw_rows(r["pbwt"], K)
+    a = run(c)
+    b = phase.phase_target(pc.params_for(c), raw, None, K, c["gt"], c["W"], r["splitsites"],
+                           r["cMdiffs"], af=c["af"], start_phase=c["start"], raw=True)
+    assert a.same(b)
+    with pytest.raises(EimpError):  # a raw row shorter than ceil(K/32) words
+        phase.phase_target(pc.params_for(c), raw[:, : (K + 31) // 32 - 1], None, K, c["gt"],
+                           c["W"], r["splitsites"], r["cMdiffs"], af=c["af"], raw=True)
+
+
 def test_rejects_bad_inputs():
     c = pc.make_case("default", 0)
     r = c["r"]
