@@ -321,7 +321,7 @@ def main():
 
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
-                          "(multi-GPU sharding is by target, see DESIGN.md)"}))
+                          "(several GPUs would each take whole targets from a shared queue)"}))
         sys.exit(2)
 
     if args.workload == 5:
