@@ -31,6 +31,21 @@ namespace eimp {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// Pointers read out of device structs are generic, so accesses through them
+// compile to FLAT instructions, which count in the LDS counter as well: the
+// next s_waitcnt for an LDS result then also waits for a FLAT store to be
+// acknowledged by memory.  Inside loops that mix LDS traffic with global
+// stores the pointers are cast to the global address space first.
+template <class T>
+using gptr_t = __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ gptr_t<T> gmem(T *p) {
+    return (gptr_t<T>)p;
+}
+// clang vector types: assignable through global-address-space pointers
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
@@ -562,11 +577,14 @@ __device__ __forceinline__ void cref_fill(const PanelDev &P, const TgtDev &T, ui
     const uint64_t *__restrict__ site = P.site + T.s0 * P.site_stride;
     const uint64_t ss = P.site_stride;
     const uint32_t S = T.S, W = T.W;
+    const gptr_t<const uint32_t> splitsites = gmem((const uint32_t *)T.splitsites);
+    const gptr_t<const uint64_t> splits = gmem((const uint64_t *)T.splits);
+    const gptr_t<const uint64_t> gt = gmem((const uint64_t *)T.gt);
     const uint32_t sa = 8 * b, sb = min(8 * b + 8, S + 1);
     const uint64_t lastmask = (P.N_ad & 63) ? ((1ull << (P.N_ad & 63)) - 1) : ~0ull;
     // sites of the block: after split sa-1 up to split sb-1 (or the window end)
-    const uint32_t m0 = sa == 0 ? 0u : T.splitsites[sa - 1] + 1;
-    const uint32_t m1 = sb <= S ? T.splitsites[sb - 1] + 1 : W;
+    const uint32_t m0 = sa == 0 ? 0u : splitsites[sa - 1] + 1;
+    const uint32_t m1 = sb <= S ? splitsites[sb - 1] + 1 : W;
     for (uint32_t wi = threadIdx.x; wi < tw; wi += 2 * CREF_NT) {
         const uint32_t wj = wi + CREF_NT;  // this thread's second word
         const bool two = wj < tw;
@@ -587,9 +605,9 @@ __device__ __forceinline__ void cref_fill(const PanelDev &P, const TgtDev &T, ui
                 const uint32_t mw = in ? m : m0;
                 v0[u] = in ? site[(uint64_t)m * ss + w] : 0ull;
                 v1[u] = in ? site[(uint64_t)m * ss + w2] : 0ull;
-                sp[u] = (uint32_t)(T.splits[mw >> 6] >> (mw & 63)) & 1u;
-                is0[u] = (uint32_t)(T.gt[2 * (mw >> 6)] >> (mw & 63)) & 1u;
-                is2[u] = (uint32_t)(T.gt[2 * (mw >> 6) + 1] >> (mw & 63)) & 1u;
+                sp[u] = (uint32_t)(splits[mw >> 6] >> (mw & 63)) & 1u;
+                is0[u] = (uint32_t)(gt[2 * (mw >> 6)] >> (mw & 63)) & 1u;
+                is2[u] = (uint32_t)(gt[2 * (mw >> 6) + 1] >> (mw & 63)) & 1u;
             }
 #pragma unroll
             for (int u = 0; u < 4; u++) {
@@ -623,8 +641,8 @@ __device__ __forceinline__ void cref_fill(const PanelDev &P, const TgtDev &T, ui
 }
 
 // first k in [lo, hi) with bh[k] >= x (bh ascending)
-__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *bh, uint32_t lo, uint32_t hi,
-                                                    uint32_t x) {
+template <class P>
+__device__ __forceinline__ uint32_t lower_bound_u32(P bh, uint32_t lo, uint32_t hi, uint32_t x) {
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
         if (bh[mid] < x) lo = mid + 1; else hi = mid;
@@ -645,12 +663,12 @@ __device__ __forceinline__ void cref_block_gather(const PanelDev &P, const TgtDe
     const uint32_t nw = (P.N_ad + 63) / 64;
     const uint32_t tw = cref_tile_words(nw);
     const bool ident = (T.flags & TF_IDENTITY) != 0;
-    const uint64_t *__restrict__ bhf = T.bhflags;
-    const uint32_t *__restrict__ bh = T.bh;
+    const gptr_t<const uint64_t> bhf = gmem((const uint64_t *)T.bhflags);
+    const gptr_t<const uint32_t> bh = gmem((const uint32_t *)T.bh);
     const uint32_t sb = min(8 * b + 8, S + 1);
     const uint32_t nrows = 2 * (sb - 8 * b) - (sb == S + 1 ? 1u : 0u);
     if (tid < 16) cnt[tid] = 0;
-    uint16_t *__restrict__ dst = T.cref + (uint64_t)b * Kpad;
+    const gptr_t<uint16_t> dst = gmem(T.cref) + (uint64_t)b * Kpad;
     for (uint32_t k = K + tid; k < Kpad; k += CREF_NT) dst[k] = 0xFFFFu;
     uint32_t ones[16];  // this thread's share of the ones among the chosen haplotypes, per row
 #pragma unroll
@@ -707,7 +725,7 @@ __device__ __forceinline__ void cref_block_gather(const PanelDev &P, const TgtDe
     __syncthreads();
     if (tid < 16) {
         const uint32_t r = 16 * b + tid;
-        if (r < Mp) T.count0[r] = (int32_t)K - (int32_t)cnt[tid];
+        if (r < Mp) gmem(T.count0)[r] = (int32_t)K - (int32_t)cnt[tid];
     }
     __syncthreads();
 }
@@ -755,7 +773,7 @@ __device__ __forceinline__ void cref_block_ident(const PanelDev &P, const TgtDev
         }
         __syncthreads();
     }
-    uint16_t *__restrict__ dst = T.cref + (uint64_t)b * Kpad;
+    const gptr_t<uint16_t> dst = gmem(T.cref) + (uint64_t)b * Kpad;
 #pragma unroll
     for (int e = 0; e < GPT; e++) {
         const uint32_t q = tid + CREF_NT * e;
@@ -768,15 +786,15 @@ __device__ __forceinline__ void cref_block_ident(const PanelDev &P, const TgtDev
                     if (k0 + 1 >= K) V[e][i] |= 0xFFFF0000u;
                 }
             }
-            uint4 *d4 = (uint4 *)(dst + 64 * q);
+            const gptr_t<u32x4> d4 = (gptr_t<u32x4>)(dst + 64 * q);
 #pragma unroll
             for (int i = 0; i < 8; i++)
-                d4[i] = make_uint4(V[e][4 * i], V[e][4 * i + 1], V[e][4 * i + 2], V[e][4 * i + 3]);
+                d4[i] = u32x4{V[e][4 * i], V[e][4 * i + 1], V[e][4 * i + 2], V[e][4 * i + 3]};
         }
     }
     if (tid < 16) {
         const uint32_t r = 16 * b + tid;
-        if (r < Mp) T.count0[r] = (int32_t)K - (int32_t)cnt[tid];
+        if (r < Mp) gmem(T.count0)[r] = (int32_t)K - (int32_t)cnt[tid];
     }
     __syncthreads();
 }
@@ -888,7 +906,7 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
     uint32_t *wzero = smem + NPOS;                 // NW: zeros of each wave's chunks
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = lane_id();
     const uint32_t ch0 = wave * NS;                // first chunk of this wave
-    const uint16_t *__restrict__ cref = T.cref;
+    const gptr_t<const uint16_t> cref = gmem((const uint16_t *)T.cref);
     const bool dig = T.digest != nullptr;
 
     uint32_t p[NS];
@@ -911,7 +929,7 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
             }
         }
     }
-    uint32_t *__restrict__ outbase = T.out + (rev ? (uint64_t)Mp * RS : 0ull);
+    const gptr_t<uint32_t> outbase = gmem(T.out) + (rev ? (uint64_t)Mp * RS : 0ull);
     const uint64_t gbase = rev ? (uint64_t)Mp * KW : 0ull;
     uint64_t dsum = 0;
     const uint32_t nslots = RS / 4;        // 4-word slots of a stored row
@@ -930,7 +948,7 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
         const uint32_t r = rev ? Mp - 1 - step : step;
         const uint32_t j = r & 15;
         if (step > 0 && (rev ? j == 15 : j == 0)) {
-            const uint32_t *__restrict__ src = (const uint32_t *)(cref + (uint64_t)(r >> 4) * Kpad);
+            const gptr_t<const uint32_t> src = (gptr_t<const uint32_t>)(cref + (uint64_t)(r >> 4) * Kpad);
             uint32_t *st32 = (uint32_t *)stage;
             __syncthreads();  // every wave has read back its payloads
             for (uint32_t q = threadIdx.x; q < Kpad / 2; q += NT) st32[q] = src[q];
@@ -944,11 +962,11 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
         }
         // pass 1: slot c's ballot words go to lane c; zeros of the wave's chunks
         uint32_t olo = 0, ohi = 0, wz = 0;
-        uint32_t bit[NS];
+        uint64_t mk[NS];  // the ballots stay in scalar registers for pass 2
         static_for<0, NS>([&](auto ci) {
             constexpr int c = decltype(ci)::value;
-            bit[c] = (p[c] >> j) & 1u;
-            const uint64_t m = __ballot(bit[c]);
+            const uint64_t m = __ballot((p[c] >> j) & 1u);
+            mk[c] = m;
             olo = writelane<c>(olo, (uint32_t)m);
             ohi = writelane<c>(ohi, (uint32_t)(m >> 32));
             wz += 64u - (uint32_t)__popcll(m);
@@ -973,21 +991,18 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
         }
         // lane c: zeros before chunk ch0 + c (lanes >= NS: don't care)
         const uint32_t zbv = run + lane_scan_excl<NS>(64u - (uint32_t)__popc(olo) - (uint32_t)__popc(ohi));
-        // pass 2: stable partition, zeros first; byte addresses into pay[]
+        // pass 2: stable partition, zeros first.  My bit is bit ln of the
+        // slot's ballot (inverse ballot: the mask itself selects); ones go to
+        // ob + (ones below me), zeros to run + (zeros below me) = run + |ln - t|
         static_for<0, NS>([&](auto ci) {
             constexpr int c = decltype(ci)::value;
-            uint32_t b = bit[c];
-            // the ballot again rather than NS masks live across the barrier
-            asm volatile("" : "+v"(b));
-            const uint64_t m = __ballot(b != 0);
-            const uint32_t zb4 = 4u * run;
-            const uint32_t ob4 = 4u * (cnt0 + (ch0 + (uint32_t)c) * 64u - run);
-            const uint32_t t4 = 4u * mbcnt(m);          // ones below me, in bytes
-            const uint32_t od4 = ob4 + t4;
-            const uint32_t zd4 = (zb4 + ln4) - t4;      // zeros below me = ln - ones
-            // unpredictable: keep a v_cndmask (a plain select may become a branch)
-            const uint32_t dest4 = __builtin_unpredictable(b != 0) ? od4 : zd4;
-            *(uint32_t *)((char *)pay + dest4) = p[c];
+            const uint64_t m = mk[c];
+            const uint32_t ob = cnt0 + (ch0 + (uint32_t)c) * 64u - run;
+            const uint32_t t = mbcnt(m);                           // ones below me
+            const uint32_t od = ob + t;
+            const uint32_t zd = __builtin_amdgcn_sad_u16(ln, t, run);
+            const uint32_t dest = __builtin_amdgcn_inverse_ballot_w64(m) ? od : zd;
+            pay[dest] = p[c];
             run += 64u - (uint32_t)__popcll(m);
         });
         __syncthreads();
@@ -995,29 +1010,29 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
         for (int c = 0; c < NS; c++) p[c] = pay[(ch0 + (uint32_t)c) * 64 + ln];
         if constexpr (ABS) {
             if (absperm) {
-                int32_t *__restrict__ ap = absperm + (uint64_t)(step + 1) * K;
+                const gptr_t<int32_t> ap = gmem(absperm) + (uint64_t)(step + 1) * K;
 #pragma unroll
                 for (int c = 0; c < NS; c++) {
                     const uint32_t pos = (ch0 + (uint32_t)c) * 64 + ln;
                     if (pos < K) ap[pos] = (int32_t)(p[c] >> 16);
                 }
             }
-            if (threadIdx.x == 0) c0out[step] = (int32_t)cnt0;
+            if (threadIdx.x == 0) gmem(c0out)[step] = (int32_t)cnt0;
         }
         // the row: chunk ch holds pairs 2ch and 2ch+1 (raw layout: words 2ch, 2ch+1)
-        uint32_t *__restrict__ row = outbase + (uint64_t)step * RS;
+        const gptr_t<uint32_t> row = outbase + (uint64_t)step * RS;
         if (ostore) {
             const uint32_t o1 = zbv + 32u - (uint32_t)__popc(olo);
             const uint32_t o3 = o1 + 32u - (uint32_t)__popc(ohi);
             const bool second = 2 * och + 1 < npairs;
             if (raw) {
                 // positions >= K are all ones: a word past the logical row is ~0
-                *(uint2 *)(row + 2 * och) = make_uint2(olo, ohi);
+                *(gptr_t<u32x2>)(row + 2 * och) = u32x2{olo, ohi};
             } else {
-                uint2 *dst = (uint2 *)(row + 4 * och);
-                dst[0] = make_uint2(olo, o1);
-                if (second) dst[1] = make_uint2(ohi, o3);
-                else if (4 * och + 2 < RS) dst[1] = make_uint2(0u, 0u);
+                const gptr_t<u32x2> dst = (gptr_t<u32x2>)(row + 4 * och);
+                dst[0] = u32x2{olo, o1};
+                if (second) dst[1] = u32x2{ohi, o3};
+                else if (4 * och + 2 < RS) dst[1] = u32x2{0u, 0u};
             }
             if (dig) {
                 // rotl(v ^ key, g mod 32) == rotr by -g: one v_alignbit each
@@ -1036,10 +1051,10 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
         // slots after the logical row: zero pairs, or all-ones words (raw)
         if (raw) {
             const uint32_t tail = nch + threadIdx.x;  // pairs of words
-            if (tail < RS / 2) ((uint2 *)row)[tail] = make_uint2(~0u, ~0u);
+            if (tail < RS / 2) ((gptr_t<u32x2>)row)[tail] = u32x2{~0u, ~0u};
         } else {
             const uint32_t tail = nch + threadIdx.x;
-            if (tail < nslots) ((uint4 *)row)[tail] = make_uint4(0u, 0u, 0u, 0u);
+            if (tail < nslots) ((gptr_t<u32x4>)row)[tail] = u32x4{0u, 0u, 0u, 0u};
         }
     }
     if (dig) {
