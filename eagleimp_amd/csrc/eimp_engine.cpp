@@ -323,12 +323,19 @@ int run_build(eimp_engine *e, uint32_t n, bool digest, uint32_t layout) {
         Kmax_pad = std::max(Kmax_pad, T.Kpad);
         const bool ident = (T.flags & TF_IDENTITY) && e->Nad <= 2ull * 256 * 64;
         if (!ident) Kmax = std::max(Kmax, T.K);
-        auto &ci = ident ? citems_id : citems;
-        for (uint32_t b = 0; b < nblk; b++) ci.push_back(CrefItem{t, b, b + 1, 0});
+        if (ident) {
+            for (uint32_t b = 0; b < nblk; b++) citems_id.push_back(CrefItem{t, b, b + 1, 0});
+        } else {
+            for (uint32_t b = 0; b < nblk; b += kCrefGroup)
+                citems.push_back(CrefItem{t, b, (uint32_t)std::min<uint64_t>(nblk, b + kCrefGroup), 0});
+        }
         pitems.push_back(PbwtItem{t, 0});
         pitems.push_back(PbwtItem{t, 1});
     }
     if (Kmax > cref_max_k()) return fail(EIMP_ERR_UNSUPPORTED, "K beyond the cref kernel limit");
+    // the subset cref kernel reads the haplotype-major panel (the drop-in's
+    // initialize uploads only the site-major one: transposed here, once)
+    if (!citems.empty()) CHK(ensure_hap(e));
     if (cref_lds_bytes(e->pd, Kmax) > 160 * 1024)
         return fail(EIMP_ERR_UNSUPPORTED, "panel too wide for the cref kernel's LDS staging");
     int v = e->choose_pbwt_variant(Kmax_pad);

@@ -90,6 +90,9 @@ struct BatchDev {
     uint32_t nrr[2];
 };
 
+// 16-row blocks per work item of the subset cref kernel (256 splits)
+constexpr uint32_t kCrefGroup = 32;
+
 // cref work item: target t, blocks [b0, b1)
 struct CrefItem {
     uint32_t t, b0, b1, pad;

@@ -42,6 +42,14 @@ template <class T>
 __device__ __forceinline__ gptr_t<T> gmem(T *p) {
     return (gptr_t<T>)p;
 }
+// Arrays that no kernel running at the same time writes, read at wave-uniform
+// addresses through the constant address space: scalar loads.
+template <class T>
+using cptr_t = const __attribute__((address_space(4))) T *;
+template <class T>
+__device__ __forceinline__ cptr_t<T> cmem(const T *p) {
+    return (cptr_t<T>)p;
+}
 // clang vector types: assignable through global-address-space pointers
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -553,17 +561,20 @@ __global__ __launch_bounds__(SEL_NT) void k_bits_to_list(const TgtDev *__restric
 }
 
 // ---------------------------------------------------------------------------
-// a7: condensed reference.  One work item = one 16-row block of a target
-// (segments 8b..8b+7).  Rows 2s (OR of the inconsistencies of segment s,
-// mergeInconsistencies, src/Target.cpp:1040-1059) and 2s+1 (the haplotypes at
-// split s, :986-1003) are built in LDS for a tile of up to CREF_TW words of the
-// haplotype axis at a time, every thread streaming two words through all the
-// block's site rows (8 independent loads in flight).  They are then reduced to
-// the K chosen haplotypes of the tile and transposed: one uint16 per haplotype
-// holds the block's 16 rows.  Haplotypes K..Kpad-1 get all-ones values (the
-// PBWT kernel's padding).
+// a7: condensed reference, in 16-row blocks (segments 8b..8b+7): row 2s is the
+// OR of the inconsistencies of segment s (mergeInconsistencies,
+// src/Target.cpp:1040-1059), row 2s+1 the haplotypes at split s (:986-1003);
+// one uint16 per haplotype holds a block's 16 rows.  Two kernels:
+//  - whole-panel targets (all N haplotypes in panel order): one work item =
+//    one block, its rows built in LDS from the site-major panel (every thread
+//    streaming two words through all the block's site rows, 8 independent
+//    loads in flight) and transposed in registers;
+//  - the others: one work item = CREF_G blocks, from the haplotype-major
+//    panel, reading only the K chosen rows.
+// count0 (zeros per row) is written by the forward PBWT pass.
 // ---------------------------------------------------------------------------
 constexpr int CREF_NT = 256;
+constexpr uint32_t CREF_G = kCrefGroup;  // blocks per work item of the subset kernel
 constexpr uint32_t CREF_TW = 512;  // words of the haplotype axis per tile (16 rows: 64 KB)
 
 __host__ __device__ inline uint32_t cref_tile_words(uint32_t nw) {
@@ -640,94 +651,97 @@ __device__ __forceinline__ void cref_fill(const PanelDev &P, const TgtDev &T, ui
     }
 }
 
-// first k in [lo, hi) with bh[k] >= x (bh ascending)
-template <class P>
-__device__ __forceinline__ uint32_t lower_bound_u32(P bh, uint32_t lo, uint32_t hi, uint32_t x) {
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (bh[mid] < x) lo = mid + 1; else hi = mid;
-    }
-    return lo;
-}
-
-// Block b of a target, K chosen haplotypes by index (any target).
-// Haplotype-outer: in each tile thread t takes the chosen haplotypes k = klo+t,
-// klo+t+CREF_NT, ... of the tile and reads the staged rows at bit bh[k]
-// (consecutive lanes read nearby words: bh is ascending), so one register
-// holds a haplotype's 16 row bits.  count0 from popcounts of the staged rows
-// under the best-hap bitmap.
-__device__ __forceinline__ void cref_block_gather(const PanelDev &P, const TgtDev &T, uint32_t b,
-                                                  uint64_t *srow, uint32_t *cnt) {
-    const uint32_t tid = threadIdx.x, l = lane_id();
-    const uint32_t K = T.K, S = T.S, Kpad = T.Kpad, Mp = T.Mp;
-    const uint32_t nw = (P.N_ad + 63) / 64;
-    const uint32_t tw = cref_tile_words(nw);
+// Blocks [b0, b1) of a subset target from the haplotype-major panel.  Thread
+// x takes the chosen haplotypes k = x, x + CREF_NT, ... and streams each one's
+// row over the sites of the blocks, 32 window sites per step:
+//   inc = (v & HomRef) | (~v & HomAlt)  (HomRef = is0 & ~is2, HomAlt =
+//                                        is2 & ~is0; split sites and sites
+//                                        outside the range masked out)
+//   C   = carries of ~splits + inc + cin
+// A split kills the carry chain, any other site propagates it and an
+// inconsistency generates it, so bit p of C says whether the segment ending
+// at split p has an inconsistency (row 2s, mergeInconsistencies,
+// src/Target.cpp:1040-1059) and bit p of v is row 2s+1 (src/Target.cpp:986-1003);
+// the carry out of bit 31 goes into the next word, and after the last word it
+// is the final segment's row 2S.  The split positions are the same for every
+// haplotype: a wave-uniform loop over them packs two bits per split into the
+// block's uint16.  Only the K chosen rows are read (the site-major layout
+// would need all N bits of every site).  Haplotypes K..Kpad-1 get all-ones
+// values (the PBWT kernel's padding); count0 comes from the PBWT kernel.
+__device__ __forceinline__ void cref_range_hap(const PanelDev &P, const TgtDev &T, uint32_t b0,
+                                               uint32_t b1) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t K = T.K, S = T.S, Kpad = T.Kpad, W = T.W;
     const bool ident = (T.flags & TF_IDENTITY) != 0;
-    const gptr_t<const uint64_t> bhf = gmem((const uint64_t *)T.bhflags);
+    // the target's arrays, written by earlier kernels and read-only here
+    const cptr_t<uint32_t> splitsites = cmem((const uint32_t *)T.splitsites);
+    const cptr_t<uint32_t> sp32 = cmem((const uint32_t *)T.splits);
+    const cptr_t<uint32_t> gt32 = cmem((const uint32_t *)T.gt);
     const gptr_t<const uint32_t> bh = gmem((const uint32_t *)T.bh);
-    const uint32_t sb = min(8 * b + 8, S + 1);
-    const uint32_t nrows = 2 * (sb - 8 * b) - (sb == S + 1 ? 1u : 0u);
-    if (tid < 16) cnt[tid] = 0;
-    const gptr_t<uint16_t> dst = gmem(T.cref) + (uint64_t)b * Kpad;
-    for (uint32_t k = K + tid; k < Kpad; k += CREF_NT) dst[k] = 0xFFFFu;
-    uint32_t ones[16];  // this thread's share of the ones among the chosen haplotypes, per row
+    const gptr_t<uint16_t> dst = gmem(T.cref);
+    const uint32_t sa = 8 * b0, sb = min(8 * b1, S + 1);
+    const bool last = sb == S + 1;  // the range ends with the final segment (no split row)
+    // window sites [m0, m1): after split sa-1, up to split sb-1 or the window end
+    const uint32_t m0 = sa == 0 ? 0u : splitsites[sa - 1] + 1;
+    const uint32_t m1 = last ? W : splitsites[sb - 1] + 1;
+    const uint32_t i0 = m0 >> 5, i1 = (m1 + 31) >> 5;  // 32-site window words
+    for (uint32_t k = K + tid; k < Kpad; k += CREF_NT)
+        for (uint32_t b = b0; b < b1; b++) dst[(uint64_t)b * Kpad + k] = 0xFFFFu;
+    // window word i of a row = bits sh.. of row words q0 + i, q0 + i + 1
+    const uint64_t rowlen = 2 * P.hap_stride;  // uint32 words per haplotype row
+    const uint64_t q0 = T.s0 >> 5;
+    const uint32_t sh = (uint32_t)T.s0 & 31;
+    for (uint32_t kb = 0; kb < K; kb += CREF_NT) {
+        const uint32_t k = kb + tid;
+        const bool act = k < K;
+        const uint32_t h = act ? (ident ? k : bh[k]) : 0u;
+        const gptr_t<const uint32_t> row = gmem((const uint32_t *)P.hap) + (uint64_t)h * rowlen;
+        uint32_t cin = 0, acc = 0, j = 0, b = b0;
+        for (uint32_t ib = i0; ib < i1; ib += 8) {
+            // the row words of eight window words, loaded together
+            uint32_t w[9];
 #pragma unroll
-    for (int rr = 0; rr < 16; rr++) ones[rr] = 0;
-    uint32_t klo = 0;  // first chosen haplotype of the tile
-    for (uint32_t w0 = 0; w0 < nw; w0 += tw) {
-        const uint32_t twc = min(tw, nw - w0);
-        cref_fill(P, T, b, w0, twc, nw, srow);
-        __syncthreads();
-        for (uint32_t wi = tid; wi < twc; wi += CREF_NT) {
-            const uint64_t f = bhf[w0 + wi];
-#pragma unroll
-            for (uint32_t rr = 0; rr < 16; rr++)
-                if (rr < nrows) ones[rr] += (uint32_t)__popcll(srow[(uint64_t)rr * twc + wi] & f);
-        }
-        const uint32_t idx_end = 64 * (w0 + twc);
-        const uint32_t khi = w0 + twc >= nw ? K : (ident ? min(K, idx_end) : lower_bound_u32(bh, klo, K, idx_end));
-        const uint32_t *rows32 = (const uint32_t *)srow;
-        const uint32_t st32 = 2 * twc;  // row stride in uint32
-        // four haplotypes per thread per round: their indices are loaded together
-        for (uint32_t kb = klo + tid; kb < khi; kb += 4 * CREF_NT) {
-            uint32_t idx[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const uint32_t k = kb + u * CREF_NT;
-                idx[u] = k < khi ? (ident ? k : bh[k]) - 64 * w0 : 0u;
+            for (int u = 0; u < 9; u++) {
+                const uint64_t q = q0 + ib + (uint32_t)u;
+                w[u] = (ib + (uint32_t)u <= i1 && q < rowlen) ? row[q] : 0u;
             }
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const uint32_t k = kb + u * CREF_NT;
-                if (k >= khi) break;
-                const uint32_t *col = rows32 + (idx[u] >> 5);
-                const uint32_t sh = idx[u] & 31;
-                uint32_t v = 0;
-                if (nrows == 16) {
-#pragma unroll
-                    for (uint32_t rr = 0; rr < 16; rr++) v |= ((col[rr * st32] >> sh) & 1u) << rr;
-                } else {
-                    for (uint32_t rr = 0; rr < nrows; rr++) v |= ((col[rr * st32] >> sh) & 1u) << rr;
+            for (int u = 0; u < 8; u++) {
+                const uint32_t i = ib + (uint32_t)u;
+                if (i >= i1) break;
+                const uint32_t v = __builtin_amdgcn_alignbit(w[u + 1], w[u], sh);
+                uint32_t rng = ~0u;
+                if (i == i0) rng &= ~0u << (m0 & 31);
+                if (i == i1 - 1 && (m1 & 31)) rng &= (1u << (m1 & 31)) - 1;
+                // the target's words are the same for every lane; a missing
+                // genotype sets both is0 and is2 and is no inconsistency
+                const uint32_t spw = __builtin_amdgcn_readfirstlane(sp32[i] & rng);
+                const uint32_t g0 = gt32[4 * (i >> 1) + (i & 1)], g2 = gt32[4 * (i >> 1) + 2 + (i & 1)];
+                const uint32_t hr = __builtin_amdgcn_readfirstlane(g0 & ~g2 & ~spw & rng);
+                const uint32_t ha = __builtin_amdgcn_readfirstlane(g2 & ~g0 & ~spw & rng);
+                const uint32_t inc = (v & hr) | (~v & ha);
+                const uint32_t pr = ~spw;
+                const uint64_t sum = (uint64_t)pr + inc + cin;
+                const uint32_t C = (uint32_t)sum ^ pr ^ inc;
+                cin = (uint32_t)(sum >> 32);
+                for (uint32_t sl = spw; sl; sl &= sl - 1) {
+                    const uint32_t p = (uint32_t)__builtin_ctz(sl);
+                    acc |= ((C >> p) & 1u) << (2 * j);
+                    acc |= ((v >> p) & 1u) << (2 * j + 1);
+                    if (++j == 8) {
+                        if (act) dst[(uint64_t)b * Kpad + k] = (uint16_t)acc;
+                        acc = 0;
+                        j = 0;
+                        b++;
+                    }
                 }
-                dst[k] = (uint16_t)v;
             }
         }
-        klo = khi;
-        __syncthreads();
-    }
-#pragma unroll
-    for (uint32_t rr = 0; rr < 16; rr++) {
-        if (rr < nrows) {
-            const uint32_t o = wave_sum(ones[rr]);
-            if (l == 0 && o) atomicAdd(&cnt[rr], o);
+        if (last) {  // row 2S: an inconsistency after the last split
+            acc |= cin << (2 * j);
+            if (act) dst[(uint64_t)b * Kpad + k] = (uint16_t)acc;
         }
     }
-    __syncthreads();
-    if (tid < 16) {
-        const uint32_t r = 16 * b + tid;
-        if (r < Mp) gmem(T.count0)[r] = (int32_t)K - (int32_t)cnt[tid];
-    }
-    __syncthreads();
 }
 
 // Block b of a target taking all N haplotypes in panel order (TF_IDENTITY):
@@ -736,16 +750,15 @@ __device__ __forceinline__ void cref_block_gather(const PanelDev &P, const TgtDe
 // contiguous bytes.
 template <int GPT>
 __device__ __forceinline__ void cref_block_ident(const PanelDev &P, const TgtDev &T, uint32_t b,
-                                                 uint64_t *srow, uint32_t *cnt) {
-    const uint32_t tid = threadIdx.x, l = lane_id();
-    const uint32_t K = T.K, S = T.S, Kpad = T.Kpad, Mp = T.Mp;
+                                                 uint64_t *srow) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t K = T.K, S = T.S, Kpad = T.Kpad;
     const uint32_t nw = (P.N_ad + 63) / 64;  // == Kpad / 64 (K == N_ad), at most CREF_TW
     uint32_t V[GPT][32];  // two uint16 values per register
 #pragma unroll
     for (int e = 0; e < GPT; e++)
 #pragma unroll
         for (int i = 0; i < 32; i++) V[e][i] = 0;
-    if (tid < 16) cnt[tid] = 0;
     const uint32_t sb = min(8 * b + 8, S + 1);
     {
         cref_fill(P, T, b, 0, nw, nw, srow);
@@ -754,12 +767,10 @@ __device__ __forceinline__ void cref_block_ident(const PanelDev &P, const TgtDev
         for (uint32_t rr = 0; rr < nrows; rr++) {
             const uint32_t rb = rr;
             const uint64_t *row = srow + (uint64_t)rr * nw;
-            uint32_t ones = 0;
 #pragma unroll
             for (int e = 0; e < GPT; e++) {
                 const uint32_t q = tid + CREF_NT * e;
                 const uint64_t wv = q < nw ? row[q] : 0ull;
-                ones += (uint32_t)__popcll(wv);
                 const uint32_t lo = (uint32_t)wv, hi = (uint32_t)(wv >> 32);
 #pragma unroll
                 for (int i = 0; i < 16; i++) {
@@ -768,8 +779,6 @@ __device__ __forceinline__ void cref_block_ident(const PanelDev &P, const TgtDev
                     V[e][16 + i] |= (((hi >> (2 * i)) & 1u) << rb) | (((hi >> (2 * i + 1)) & 1u) << (16 + rb));
                 }
             }
-            ones = wave_sum(ones);
-            if (l == 0) atomicAdd(&cnt[rb], ones);
         }
         __syncthreads();
     }
@@ -792,28 +801,20 @@ __device__ __forceinline__ void cref_block_ident(const PanelDev &P, const TgtDev
                 d4[i] = u32x4{V[e][4 * i], V[e][4 * i + 1], V[e][4 * i + 2], V[e][4 * i + 3]};
         }
     }
-    if (tid < 16) {
-        const uint32_t r = 16 * b + tid;
-        if (r < Mp) gmem(T.count0)[r] = (int32_t)K - (int32_t)cnt[tid];
-    }
-    __syncthreads();
 }
 
 __global__ __launch_bounds__(CREF_NT) void k_cref(PanelDev P, const TgtDev *__restrict__ tg,
                                                  const CrefItem *__restrict__ items) {
-    extern __shared__ uint64_t srow[];
-    __shared__ uint32_t cnt[16];
     const CrefItem it = items[blockIdx.x];
-    for (uint32_t b = it.b0; b < it.b1; b++) cref_block_gather(P, tg[it.t], b, srow, cnt);
+    cref_range_hap(P, tg[it.t], it.b0, it.b1);
 }
 
 template <int GPT>
 __global__ __launch_bounds__(CREF_NT) void k_cref_ident(PanelDev P, const TgtDev *__restrict__ tg,
                                                        const CrefItem *__restrict__ items) {
     extern __shared__ uint64_t srow[];
-    __shared__ uint32_t cnt[16];
     const CrefItem it = items[blockIdx.x];
-    for (uint32_t b = it.b0; b < it.b1; b++) cref_block_ident<GPT>(P, tg[it.t], b, srow, cnt);
+    for (uint32_t b = it.b0; b < it.b1; b++) cref_block_ident<GPT>(P, tg[it.t], b, srow);
 }
 
 // ---------------------------------------------------------------------------
@@ -936,7 +937,6 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
     const bool raw = (T.flags & TF_RAW) != 0;
     const uint32_t och = ch0 + ln;         // chunk stored by this lane
     const bool ostore = ln < (uint32_t)NS && och < nch;
-    const uint32_t ln4 = 4u * ln;
     // digest of this lane's words: g = gbase + step*KW + 4*och (+0..3); the
     // key g * kDigKey and the rotation -g (alignbit uses 5 bits) advance by
     // a constant per step
@@ -988,6 +988,11 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
             const uint32_t wex = lane_scan_excl<NW>(wv);
             run = (uint32_t)__builtin_amdgcn_readlane((int)wex, (int)wave);
             cnt0 = (uint32_t)__builtin_amdgcn_readlane((int)(wex + wv), NW - 1);
+        }
+        if constexpr (!ABS) {
+            // count0 of the row: its zeros among the K chosen haplotypes (the
+            // padding positions are all ones); the forward pass writes it
+            if (!rev && threadIdx.x == 0) gmem(T.count0)[r] = (int32_t)cnt0;
         }
         // lane c: zeros before chunk ch0 + c (lanes >= NS: don't care)
         const uint32_t zbv = run + lane_scan_excl<NS>(64u - (uint32_t)__popc(olo) - (uint32_t)__popc(ohi));
@@ -1048,13 +1053,18 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
         }
         dkey += dkey_step;
         dneg -= KW;
-        // slots after the logical row: zero pairs, or all-ones words (raw)
+        // slots after the logical row: zero pairs, or all-ones words (raw).
+        // The fill value is made here (asm: not hoisted out of the loop,
+        // where the constant vector was kept in scratch at tight register
+        // budgets).
+        uint32_t fill;
+        asm volatile("v_mov_b32 %0, 0" : "=v"(fill));
         if (raw) {
             const uint32_t tail = nch + threadIdx.x;  // pairs of words
-            if (tail < RS / 2) ((gptr_t<u32x2>)row)[tail] = u32x2{~0u, ~0u};
+            if (tail < RS / 2) ((gptr_t<u32x2>)row)[tail] = u32x2{~fill, ~fill};
         } else {
             const uint32_t tail = nch + threadIdx.x;
-            if (tail < nslots) ((gptr_t<u32x4>)row)[tail] = u32x4{0u, 0u, 0u, 0u};
+            if (tail < nslots) ((gptr_t<u32x4>)row)[tail] = u32x4{fill, fill, fill, fill};
         }
     }
     if (dig) {
@@ -1088,7 +1098,7 @@ __device__ __forceinline__ void pbwt_target(const TgtDev &T, const bool rev, uin
 // Waves per SIMD that one workgroup's LDS allows at the variant's capacity
 // (at most 6, above which the larger variants would spill registers): the
 // register budget is capped so that VGPRs do not lower the occupancy (512 x 20:
-// 80 VGPRs, three workgroups per CU instead of two at 86).
+// three workgroups per CU need at most 85 VGPRs).
 constexpr int pbwt_waves_hint(int NT, int CPW) {
     const int lds = CPW * NT * 4 + (NT / 64) * 4;
     const int w = (163840 / lds) * (NT / 256);
@@ -1260,12 +1270,14 @@ __global__ __launch_bounds__(PLAN_NT) void k_plan(TgtDev *__restrict__ tg, Batch
     // panel rows at the same time, from the cache.
     const bool idk = fit && A.ident_ok && (tg[t].flags & TF_IDENTITY);
     const bool gk = fit && !idk;
+    // subset targets are queued in items of CREF_G blocks
+    const uint32_t ngrp = (nblk + CREF_G - 1) / CREF_G;
     uint64_t nb_tot, nb_id, ng, ni, mxg, mxi;
-    const uint64_t bp = block_excl_scan64(gk ? nblk : 0u, s_w, nb_tot);
+    const uint64_t bp = block_excl_scan64(gk ? ngrp : 0u, s_w, nb_tot);
     const uint64_t bpi = block_excl_scan64(idk ? nblk : 0u, s_w, nb_id);
     const uint64_t pg = block_excl_scan64(gk ? 1u : 0u, s_w, ng);
     const uint64_t pi = block_excl_scan64(idk ? 1u : 0u, s_w, ni);
-    mxg = block_max64(gk ? nblk : 0u, s_w);
+    mxg = block_max64(gk ? ngrp : 0u, s_w);
     mxi = block_max64(idk ? nblk : 0u, s_w);
     const bool rrg = ng > 1 && ng * mxg * 4 <= nb_tot * 5;
     const bool rri = ni > 1 && ni * mxi * 4 <= nb_id * 5;
@@ -1327,12 +1339,13 @@ __global__ __launch_bounds__(PLAN_NT) void k_plan(TgtDev *__restrict__ tg, Batch
     }
 }
 
-// next cref block from a queue: queue position i (i >= nq ends), target t,
-// block b of t; b = ~0u for an empty item of a block-major queue
+// next cref item from a queue: queue position i (i >= nq ends), target t,
+// item b of t (items of `unit` 16-row blocks); b = ~0u for an empty item of a
+// block-major queue
 __device__ __forceinline__ void cref_pop(uint32_t *ctr, uint32_t nq, uint32_t n, uint32_t rr,
                                          uint32_t nrr, const TgtDev *__restrict__ tg,
-                                         const uint32_t *__restrict__ pref, uint32_t &i,
-                                         uint32_t &t, uint32_t &b) {
+                                         const uint32_t *__restrict__ pref, uint32_t unit,
+                                         uint32_t &i, uint32_t &t, uint32_t &b) {
     __shared__ uint32_t s_i, s_t, s_b;
     if (threadIdx.x == 0) {
         const uint32_t ii = atomicAdd(ctr, 1u);
@@ -1341,7 +1354,7 @@ __device__ __forceinline__ void cref_pop(uint32_t *ctr, uint32_t nq, uint32_t n,
             if (rr) {  // block-major: block ii / nrr of the (ii % nrr)-th listed target
                 tt = pref[ii % nrr];
                 bb = ii / nrr;
-                if (bb >= (tg[tt].Mp + 15) / 16) bb = ~0u;
+                if (bb >= ((tg[tt].Mp + 15) / 16 + unit - 1) / unit) bb = ~0u;
             } else {  // last target whose first block is <= ii (it owns block ii)
                 uint32_t lo = 0, hi = n;
                 while (hi - lo > 1) {
@@ -1357,24 +1370,27 @@ __device__ __forceinline__ void cref_pop(uint32_t *ctr, uint32_t nq, uint32_t n,
         s_b = bb;
     }
     __syncthreads();
-    i = s_i;
-    t = s_t;
-    b = s_b;
+    // the same for every lane: scalar registers, so the target's fields load
+    // through the scalar cache
+    i = __builtin_amdgcn_readfirstlane(s_i);
+    t = __builtin_amdgcn_readfirstlane(s_t);
+    b = __builtin_amdgcn_readfirstlane(s_b);
     __syncthreads();
 }
 
-// cref blocks of subset targets from their queue
+// cref items (CREF_G blocks) of subset targets from their queue
 __global__ __launch_bounds__(CREF_NT) void k_cref_persist(PanelDev P, const TgtDev *__restrict__ tg,
                                                          BatchDev *__restrict__ B,
                                                          const uint32_t *__restrict__ blockpref) {
-    extern __shared__ uint64_t srow[];
-    __shared__ uint32_t cnt[16];
     const uint32_t nq = B->ncref, n = B->n, rr = B->rr[0], nrr = B->nrr[0];
     for (;;) {
-        uint32_t i, t, b;
-        cref_pop(&B->cref_ctr, nq, n, rr, nrr, tg, blockpref, i, t, b);
+        uint32_t i, t, g;
+        cref_pop(&B->cref_ctr, nq, n, rr, nrr, tg, blockpref, CREF_G, i, t, g);
         if (i >= nq) break;
-        if (b != ~0u) cref_block_gather(P, tg[t], b, srow, cnt);
+        if (g != ~0u) {
+            const uint32_t nblk = (tg[t].Mp + 15) / 16;
+            cref_range_hap(P, tg[t], g * CREF_G, min(nblk, (g + 1) * CREF_G));
+        }
     }
 }
 
@@ -1385,13 +1401,12 @@ __global__ __launch_bounds__(CREF_NT, 4) void k_cref_persist_ident(PanelDev P,
                                                                BatchDev *__restrict__ B,
                                                                const uint32_t *__restrict__ blockpref) {
     extern __shared__ uint64_t srow[];
-    __shared__ uint32_t cnt[16];
     const uint32_t nq = B->ncref_id, n = B->n, rr = B->rr[1], nrr = B->nrr[1];
     for (;;) {
         uint32_t i, t, b;
-        cref_pop(&B->cref_id_ctr, nq, n, rr, nrr, tg, blockpref, i, t, b);
+        cref_pop(&B->cref_id_ctr, nq, n, rr, nrr, tg, blockpref, 1u, i, t, b);
         if (i >= nq) break;
-        if (b != ~0u) cref_block_ident<GPT>(P, tg[t], b, srow, cnt);
+        if (b != ~0u) cref_block_ident<GPT>(P, tg[t], b, srow);
     }
 }
 
@@ -1525,9 +1540,8 @@ hipError_t launch_cref(const PanelDev &P, const TgtDev *tg, const CrefItem *item
             return hipGetLastError();
         }
     }
-    hipError_t e = raise_lds_limit((const void *)k_cref, lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_cref, dim3(nitems), dim3(CREF_NT), lds, s, P, tg, items);
+    if (!P.hap || !P.hap_stride) return hipErrorInvalidValue;  // the haplotype-major panel is needed
+    hipLaunchKernelGGL(k_cref, dim3(nitems), dim3(CREF_NT), 0, s, P, tg, items);
     return hipGetLastError();
 }
 
@@ -1658,10 +1672,11 @@ bool cref_ident_ok(const PanelDev &P) { return (P.N_ad + 63) / 64 <= 2u * CREF_N
 // sets the dynamic LDS limit; occ[0], occ[1] = resident workgroups per CU of
 // the subset and identity kernels
 hipError_t cref_persist_prepare(const PanelDev &P, uint32_t maxK, int *occ) {
-    const size_t lds = cref_lds_bytes(P, maxK);
     for (int k = 0; k < 2; k++) {
         occ[k] = 0;
         if (k == 1 && !cref_ident_ok(P)) continue;
+        // the subset kernel stages nothing in LDS
+        const size_t lds = k == 1 ? cref_lds_bytes(P, maxK) : 0;
         const void *fn = (const void *)cref_persist_fn(P, k == 1);
         hipError_t e = raise_lds_limit(fn, lds);
         if (e != hipSuccess) return e;
@@ -1675,8 +1690,9 @@ hipError_t launch_cref_persist(const PanelDev &P, const TgtDev *tg, BatchDev *B,
                                const uint32_t *blockpref, const uint32_t *blockpref_id,
                                uint32_t maxK, const uint32_t *grid, hipStream_t s) {
     const size_t lds = cref_lds_bytes(P, maxK);
+    if (grid[0] && (!P.hap || !P.hap_stride)) return hipErrorInvalidValue;
     if (grid[0])
-        hipLaunchKernelGGL(cref_persist_fn(P, false), dim3(grid[0]), dim3(CREF_NT), lds, s, P, tg,
+        hipLaunchKernelGGL(cref_persist_fn(P, false), dim3(grid[0]), dim3(CREF_NT), 0, s, P, tg,
                            B, blockpref);
     if (grid[1] && cref_ident_ok(P))
         hipLaunchKernelGGL(cref_persist_fn(P, true), dim3(grid[1]), dim3(CREF_NT), lds, s, P, tg, B,
