@@ -191,8 +191,8 @@ def test_pbwt_variants_agree(built, small, variant):
 
 
 def test_wide_panel_cref_tiles(built):
-    """N > 32,768 haplotypes: the cref kernels stage the haplotype axis in
-    several tiles per block (subset targets only: K is at most 24,576)."""
+    """N > 32,768 haplotypes: whole-panel targets leave the transpose kernel
+    for the haplotype-major one, subset targets take K up to 24,000."""
     panel = synth.make_panel(17, N=33500, M=700, F=40, T=6)
     pv = oracle.PanelView(panel.hap, panel.site, panel.cM)
     eng = _panel_engine(panel)
@@ -207,5 +207,47 @@ def test_wide_panel_cref_tiles(built):
         r = oracle.process_target(pv, tg.gt[t], tg.s0[t], tg.W[t], tg.K[t], keep=False)
         assert (int(res.Mp[t]), int(res.dig_pbwt[t]), int(res.dig_count0[t])) == \
             (r["Mp"], r["dig_pbwt"], r["dig_count0"]), f"stream wide target {t}"
+    st.close()
+    eng.close()
+
+
+def test_subset_cref_edge_windows(built, small):
+    """Subset targets through the haplotype-major cref kernel, windows chosen
+    for its edge cases: no split at all (S = 0), every S mod 8 (S % 8 == 0
+    leaves the last 16-row block holding only the final segment), windows
+    ending at the panel's last site, unaligned starts and all-missing
+    stretches (is0 and is2 both set: no inconsistency); synchronous engine
+    and stream context."""
+    from phase_cases import encode_gt
+    panel = small
+    pv = oracle.PanelView(panel.hap, panel.site, panel.cM)
+    M = panel.M
+    bits = lambda w: np.unpackbits(w.view(np.uint8), bitorder="little")[:M]  # noqa: E731
+    codes = (bits(panel.tgt_hap[0]).astype(np.uint8) + bits(panel.tgt_hap[1])).astype(np.uint8)
+    # 0 HomRef, 1 Het, 2 HomAlt
+    rs = np.random.default_rng(5)
+    cases = [(100, 50, 150, "nohet"), (3940, 60, 150, None), (3999, 1, 64, None),
+             (M - 333, 333, 200, "miss"), (5, 700, 299, "miss")]
+    cases += [(17 + 7 * i, 40 + 3 * i, 100 + i, None) for i in range(30)]
+    gts, s0s, Ws, Ks = [], [], [], []
+    for s0, W, K, mod in cases:
+        c = codes[s0:s0 + W].copy()
+        if mod == "nohet":
+            c[c == 1] = 0
+        elif mod == "miss":
+            a = int(rs.integers(0, max(1, W - 40)))
+            c[a:a + 40] = 3
+        gts.append(encode_gt(c)); s0s.append(s0); Ws.append(W); Ks.append(K)
+    eng = _panel_engine(panel)
+    res = _run_and_compare(panel, pv, eng, gts, s0s, Ws, Ks, tag="cref edges")
+    S = [int(x) for x in res.S]
+    assert 0 in S and {s % 8 for s in S if s} == set(range(8)), f"edge classes not reached: {S}"
+    st = eng.stream(len(gts), max(Ws), max(Ks), 1 << 30)
+    st.submit(gts, s0s, Ws, Ks)
+    sres, nd, _ = st.wait()
+    assert nd == 0
+    for t in range(len(gts)):
+        assert (int(sres.Mp[t]), int(sres.dig_pbwt[t]), int(sres.dig_count0[t])) == \
+            (int(res.Mp[t]), int(res.dig_pbwt[t]), int(res.dig_count0[t])), f"stream target {t}"
     st.close()
     eng.close()
