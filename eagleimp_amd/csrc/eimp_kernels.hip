@@ -50,6 +50,15 @@ template <class T>
 __device__ __forceinline__ cptr_t<T> cmem(const T *p) {
     return (cptr_t<T>)p;
 }
+// compile-time loop: f(std::integral_constant<int, I>) for I in [B, E)
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
 // clang vector types: assignable through global-address-space pointers
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -744,9 +753,30 @@ __device__ __forceinline__ void cref_range_hap(const PanelDev &P, const TgtDev &
     }
 }
 
+// In-register transpose of two 16x16 bit blocks side by side: on entry bit c
+// of a[r] is (row r, column c) for c < 16 and (row r, column 16 + c') at bit
+// 16 + c'; on exit bit r of a[c] is (row r, column c) and bit 16 + r is
+// (row r, column 16 + c).  Four rounds of masked block swaps (bits of one
+// 16-bit half never move into the other: the masks repeat per half).
+__device__ __forceinline__ void transpose16x2(uint32_t (&a)[16]) {
+    static_for<0, 4>([&](auto ji) {
+        constexpr int j = 8 >> decltype(ji)::value;
+        constexpr uint32_t m = j == 8 ? 0x00FF00FFu : j == 4 ? 0x0F0F0F0Fu : j == 2 ? 0x33333333u
+                                                                                     : 0x55555555u;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            if (k & j) continue;
+            const uint32_t t = ((a[k] >> j) ^ a[k + j]) & m;
+            a[k + j] ^= t;
+            a[k] ^= t << j;
+        }
+    });
+}
+
 // Block b of a target taking all N haplotypes in panel order (TF_IDENTITY):
-// no gather.  Each thread owns GPT groups of 64 haplotypes, reads their word of
-// every staged row and transposes into 64 uint16 values, written as 128
+// no gather.  Each thread owns GPT groups of 64 haplotypes: it reads their
+// word of the 16 staged rows, transposes the low and the high halves as two
+// pairs of 16x16 bit blocks, and writes the 64 uint16 values as 128
 // contiguous bytes.
 template <int GPT>
 __device__ __forceinline__ void cref_block_ident(const PanelDev &P, const TgtDev &T, uint32_t b,
@@ -754,34 +784,33 @@ __device__ __forceinline__ void cref_block_ident(const PanelDev &P, const TgtDev
     const uint32_t tid = threadIdx.x;
     const uint32_t K = T.K, S = T.S, Kpad = T.Kpad;
     const uint32_t nw = (P.N_ad + 63) / 64;  // == Kpad / 64 (K == N_ad), at most CREF_TW
-    uint32_t V[GPT][32];  // two uint16 values per register
-#pragma unroll
-    for (int e = 0; e < GPT; e++)
-#pragma unroll
-        for (int i = 0; i < 32; i++) V[e][i] = 0;
     const uint32_t sb = min(8 * b + 8, S + 1);
-    {
-        cref_fill(P, T, b, 0, nw, nw, srow);
-        __syncthreads();
-        const uint32_t nrows = 2 * (sb - 8 * b) - (sb == S + 1 ? 1u : 0u);
-        for (uint32_t rr = 0; rr < nrows; rr++) {
-            const uint32_t rb = rr;
-            const uint64_t *row = srow + (uint64_t)rr * nw;
+    cref_fill(P, T, b, 0, nw, nw, srow);
+    __syncthreads();
+    const uint32_t nrows = 2 * (sb - 8 * b) - (sb == S + 1 ? 1u : 0u);
+    uint32_t V[GPT][32];  // two uint16 values per register: haplotypes 2i, 2i+1
 #pragma unroll
-            for (int e = 0; e < GPT; e++) {
-                const uint32_t q = tid + CREF_NT * e;
-                const uint64_t wv = q < nw ? row[q] : 0ull;
-                const uint32_t lo = (uint32_t)wv, hi = (uint32_t)(wv >> 32);
+    for (int e = 0; e < GPT; e++) {
+        const uint32_t q = tid + CREF_NT * e;
+        uint32_t lo[16], hi[16];
 #pragma unroll
-                for (int i = 0; i < 16; i++) {
-                    // haplotypes 2i, 2i+1 (low word) and 32+2i, 33+2i (high word)
-                    V[e][i] |= (((lo >> (2 * i)) & 1u) << rb) | (((lo >> (2 * i + 1)) & 1u) << (16 + rb));
-                    V[e][16 + i] |= (((hi >> (2 * i)) & 1u) << rb) | (((hi >> (2 * i + 1)) & 1u) << (16 + rb));
-                }
-            }
+        for (int rr = 0; rr < 16; rr++) {
+            const uint64_t wv = (q < nw && (uint32_t)rr < nrows) ? srow[(uint64_t)rr * nw + q] : 0ull;
+            lo[rr] = (uint32_t)wv;
+            hi[rr] = (uint32_t)(wv >> 32);
         }
-        __syncthreads();
+        transpose16x2(lo);
+        transpose16x2(hi);
+        // lo[c]: haplotype c in the low half, 16 + c in the high half
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            V[e][i] = __builtin_amdgcn_perm(lo[2 * i + 1], lo[2 * i], 0x05040100u);
+            V[e][8 + i] = __builtin_amdgcn_perm(lo[2 * i + 1], lo[2 * i], 0x07060302u);
+            V[e][16 + i] = __builtin_amdgcn_perm(hi[2 * i + 1], hi[2 * i], 0x05040100u);
+            V[e][24 + i] = __builtin_amdgcn_perm(hi[2 * i + 1], hi[2 * i], 0x07060302u);
+        }
     }
+    __syncthreads();  // the staged rows are free for the next block
     const gptr_t<uint16_t> dst = gmem(T.cref) + (uint64_t)b * Kpad;
 #pragma unroll
     for (int e = 0; e < GPT; e++) {
@@ -853,14 +882,6 @@ __device__ __forceinline__ uint32_t writelane(uint32_t old, uint32_t val) {
     return eimp_llvm_writelane(val, (uint32_t)L, old);
 }
 
-// compile-time loop: f(std::integral_constant<int, I>) for I in [B, E)
-template <int B, int E, typename F>
-__device__ __forceinline__ void static_for(F &&f) {
-    if constexpr (B < E) {
-        f(std::integral_constant<int, B>{});
-        static_for<B + 1, E>(f);
-    }
-}
 
 // Exclusive prefix sum of v over lanes 0..N-1 (other lanes: garbage), by DPP
 // row shifts and row broadcasts (no LDS).
