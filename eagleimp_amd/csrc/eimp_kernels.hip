@@ -706,13 +706,25 @@ __device__ __forceinline__ void cref_range_hap(const PanelDev &P, const TgtDev &
         const uint32_t h = act ? (ident ? k : bh[k]) : 0u;
         const gptr_t<const uint32_t> row = gmem((const uint32_t *)P.hap) + (uint64_t)h * rowlen;
         uint32_t cin = 0, acc = 0, j = 0, b = b0;
-        for (uint32_t ib = i0; ib < i1; ib += 8) {
-            // the row words of eight window words, loaded together
-            uint32_t w[9];
+        // row word x (x <= i1, inside the row) of window word x
+        auto rword = [&](uint32_t x) -> uint32_t {
+            const uint64_t q = q0 + x;
+            return (x <= i1 && q < rowlen) ? row[q] : 0u;
+        };
+        // the row words of eight window words are loaded together, one batch
+        // ahead of the words being processed
+        uint32_t w8 = rword(i0), nx[8];
 #pragma unroll
-            for (int u = 0; u < 9; u++) {
-                const uint64_t q = q0 + ib + (uint32_t)u;
-                w[u] = (ib + (uint32_t)u <= i1 && q < rowlen) ? row[q] : 0u;
+        for (int u = 0; u < 8; u++) nx[u] = rword(i0 + 1 + (uint32_t)u);
+        for (uint32_t ib = i0; ib < i1; ib += 8) {
+            uint32_t w[9];
+            w[0] = w8;
+#pragma unroll
+            for (int u = 0; u < 8; u++) w[u + 1] = nx[u];
+            w8 = w[8];
+            if (ib + 8 < i1) {
+#pragma unroll
+                for (int u = 0; u < 8; u++) nx[u] = rword(ib + 9 + (uint32_t)u);
             }
 #pragma unroll
             for (int u = 0; u < 8; u++) {
