@@ -166,19 +166,19 @@ def make_arrivals(seed: int, n: int, rate: float, npool: int, bmin: int = 1, bma
 # Benchmark workloads.  Site spacing is 1 site/kb with a linear 1 cM/Mb map
 # for every panel; the seed of each workload is its number.
 WORKLOADS = {
-    1: dict(name="cfg1-cpu-2kx10k-K1000", N=2000, M=10000, F=50, beta_a=0.5, T=20,
+    1: dict(name="cpu-2kx10k-K1000", N=2000, M=10000, F=50, beta_a=0.5, T=20,
             K=1000, miss=0.01),
-    2: dict(name="cfg2-plain-10kx100k-K10000", N=10000, M=100000, F=200, beta_a=0.5, T=500,
+    2: dict(name="plain-10kx100k-K10000", N=10000, M=100000, F=200, beta_a=0.5, T=500,
             K=10000, miss=0.01),
-    3: dict(name="cfg3-batched-20kx200k-window-K2k..10k", N=20000, M=200000, F=200, beta_a=0.5,
+    3: dict(name="batched-20kx200k-window-K2k..10k", N=20000, M=200000, F=200, beta_a=0.5,
             T=5000, kset=[2000, 4000, 6000, 8000, 10000], wmin=20000, wmax=200000,
             miss=0.0, miss_hi=0.05),
-    4: dict(name="cfg4-large-200kx500k-K10000", N=200000, M=500000, F=2000, beta_a=0.3,
+    4: dict(name="large-200kx500k-K10000", N=200000, M=500000, F=2000, beta_a=0.3,
             T=10000, K=10000, miss=0.01),
     # serving: the 20k x 200k panel (panel seed 3) resident, a pool of targets
     # with the windows / K / missingness of the batched workload, request
     # batches of 1-256 targets arriving at 20-200 batches/s
-    5: dict(name="cfg5-serving-20kx200k-poisson", panel_seed=3, N=20000, M=200000, F=200,
+    5: dict(name="serving-20kx200k-poisson", panel_seed=3, N=20000, M=200000, F=200,
             beta_a=0.5, T=1024, kset=[2000, 4000, 6000, 8000, 10000], wmin=20000,
             wmax=200000, miss=0.0, miss_hi=0.05, rates=(20.0, 200.0), bmin=1, bmax=256),
 }
