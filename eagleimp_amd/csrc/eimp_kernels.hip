@@ -941,7 +941,8 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = lane_id();
     const uint32_t ch0 = wave * NS;                // first chunk of this wave
     const gptr_t<const uint16_t> cref = gmem((const uint16_t *)T.cref);
-    const bool dig = T.digest != nullptr;
+    // wave-uniform flags as scalars: their branches stay scalar branches
+    const bool dig = __builtin_amdgcn_readfirstlane(T.digest != nullptr ? 1u : 0u) != 0;
 
     uint32_t p[NS];
     {
@@ -966,17 +967,24 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
     const gptr_t<uint32_t> outbase = gmem(T.out) + (rev ? (uint64_t)Mp * RS : 0ull);
     const uint64_t gbase = rev ? (uint64_t)Mp * KW : 0ull;
     uint64_t dsum = 0;
-    const uint32_t nslots = RS / 4;        // 4-word slots of a stored row
-    const bool raw = (T.flags & TF_RAW) != 0;
+    const bool raw = (__builtin_amdgcn_readfirstlane(T.flags) & TF_RAW) != 0;
     const uint32_t och = ch0 + ln;         // chunk stored by this lane
     const bool ostore = ln < (uint32_t)NS && och < nch;
-    // digest of this lane's words: g = gbase + step*KW + 4*och (+0..3); the
-    // key g * kDigKey and the rotation -g (alignbit uses 5 bits) advance by
-    // a constant per step
+    // digest of this lane's words: g = gbase + step*KW + 4*och (+0..3), key
+    // g * kDigKey, rotation -g (alignbit uses 5 bits)
     const uint32_t g0 = (uint32_t)gbase + 4u * och;
-    uint32_t dkey = g0 * kDigKey, dneg = 0u - g0;
-    const uint32_t dkey_step = KW * kDigKey;
 
+    {
+        // The padding words of every row of this direction, stored once: after
+        // the logical pairs (words >= 2*npairs) zero, or in the raw layout
+        // all ones after the data words (positions >= K are all ones).
+        const uint32_t pw0 = raw ? 2 * nch : 2 * npairs;  // even, as RS
+        const uint32_t fw = raw ? ~0u : 0u;
+        for (uint32_t rr = threadIdx.x; rr < Mp; rr += NT) {
+            const gptr_t<u32x2> prow = (gptr_t<u32x2>)(outbase + (uint64_t)rr * RS);
+            for (uint32_t w = pw0 / 2; w < RS / 2; w++) prow[w] = u32x2{fw, fw};
+        }
+    }
     for (uint32_t step = 0; step < Mp; ++step) {
         const uint32_t r = rev ? Mp - 1 - step : step;
         const uint32_t j = r & 15;
@@ -993,8 +1001,10 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
                 p[c] = (id << 16) | (id < Kpad ? (uint32_t)stage[id] : 0xFFFFu);
             }
         }
-        // pass 1: slot c's ballot words go to lane c; zeros of the wave's chunks
-        uint32_t olo = 0, ohi = 0, wz = 0;
+        // pass 1: slot c's ballot words go to lane c; zeros of the wave's chunks.
+        // Lanes >= NS of olo/ohi are never read: no initial value (asm).
+        uint32_t olo, ohi, wz = 0;
+        asm volatile("" : "=v"(olo), "=v"(ohi));
         uint64_t mk[NS];  // the ballots stay in scalar registers for pass 2
         static_for<0, NS>([&](auto ci) {
             constexpr int c = decltype(ci)::value;
@@ -1059,45 +1069,31 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
         }
         // the row: chunk ch holds pairs 2ch and 2ch+1 (raw layout: words 2ch, 2ch+1)
         const gptr_t<uint32_t> row = outbase + (uint64_t)step * RS;
+        const uint32_t o1 = zbv + 32u - (uint32_t)__popc(olo);
+        const uint32_t o3 = o1 + 32u - (uint32_t)__popc(ohi);
+        const bool second = 2 * och + 1 < npairs;
+        // the logical words only: the padding was filled before the loop
         if (ostore) {
-            const uint32_t o1 = zbv + 32u - (uint32_t)__popc(olo);
-            const uint32_t o3 = o1 + 32u - (uint32_t)__popc(ohi);
-            const bool second = 2 * och + 1 < npairs;
             if (raw) {
-                // positions >= K are all ones: a word past the logical row is ~0
                 *(gptr_t<u32x2>)(row + 2 * och) = u32x2{olo, ohi};
             } else {
                 const gptr_t<u32x2> dst = (gptr_t<u32x2>)(row + 4 * och);
                 dst[0] = u32x2{olo, o1};
                 if (second) dst[1] = u32x2{ohi, o3};
-                else if (4 * och + 2 < RS) dst[1] = u32x2{0u, 0u};
-            }
-            if (dig) {
-                // rotl(v ^ key, g mod 32) == rotr by -g: one v_alignbit each
-                dsum += __builtin_amdgcn_alignbit(olo ^ dkey, olo ^ dkey, dneg);
-                const uint32_t x1 = o1 ^ (dkey + kDigKey);
-                dsum += __builtin_amdgcn_alignbit(x1, x1, dneg - 1u);
-                if (second) {
-                    const uint32_t x2 = ohi ^ (dkey + 2u * kDigKey), x3 = o3 ^ (dkey + 3u * kDigKey);
-                    dsum += __builtin_amdgcn_alignbit(x2, x2, dneg - 2u);
-                    dsum += __builtin_amdgcn_alignbit(x3, x3, dneg - 3u);
-                }
             }
         }
-        dkey += dkey_step;
-        dneg -= KW;
-        // slots after the logical row: zero pairs, or all-ones words (raw).
-        // The fill value is made here (asm: not hoisted out of the loop,
-        // where the constant vector was kept in scratch at tight register
-        // budgets).
-        uint32_t fill;
-        asm volatile("v_mov_b32 %0, 0" : "=v"(fill));
-        if (raw) {
-            const uint32_t tail = nch + threadIdx.x;  // pairs of words
-            if (tail < RS / 2) ((gptr_t<u32x2>)row)[tail] = u32x2{~fill, ~fill};
-        } else {
-            const uint32_t tail = nch + threadIdx.x;
-            if (tail < nslots) ((gptr_t<u32x4>)row)[tail] = u32x4{fill, fill, fill, fill};
+        if (dig && ostore) {
+            // rotl(v ^ key, g mod 32) == rotr by -g: one v_alignbit each
+            const uint32_t g = g0 + step * KW;
+            const uint32_t dkey = g * kDigKey, dneg = 0u - g;
+            dsum += __builtin_amdgcn_alignbit(olo ^ dkey, olo ^ dkey, dneg);
+            const uint32_t x1 = o1 ^ (dkey + kDigKey);
+            dsum += __builtin_amdgcn_alignbit(x1, x1, dneg - 1u);
+            if (second) {
+                const uint32_t x2 = ohi ^ (dkey + 2u * kDigKey), x3 = o3 ^ (dkey + 3u * kDigKey);
+                dsum += __builtin_amdgcn_alignbit(x2, x2, dneg - 2u);
+                dsum += __builtin_amdgcn_alignbit(x3, x3, dneg - 3u);
+            }
         }
     }
     if (dig) {
