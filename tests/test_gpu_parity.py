@@ -190,7 +190,7 @@ def test_pbwt_variants_agree(built, small, variant):
     eng.close()
 
 
-def test_wide_panel_cref_tiles(built):
+def test_wide_panel_cref(built):
     """N > 32,768 haplotypes: whole-panel targets leave the transpose kernel
     for the haplotype-major one, subset targets take K up to 24,000."""
     panel = synth.make_panel(17, N=33500, M=700, F=40, T=6)
