@@ -934,8 +934,9 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
     const uint32_t npairs = (K + 31) / 32;
     const uint32_t KW = 2 * npairs;
     uint32_t *pay = smem;                          // NPOS payloads
-    // Kpad uint16: the next cref block by haplotype, staged at block boundaries
-    // (when no payloads are in flight) in the first half of pay[]
+    // Kpad + 2 uint16: the next cref block by haplotype (and all ones at id
+    // Kpad), staged at block boundaries (when no payloads are in flight) in
+    // the first half of pay[]
     uint16_t *stage = (uint16_t *)smem;
     uint32_t *wzero = smem + NPOS;                 // NW: zeros of each wave's chunks
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = lane_id();
@@ -950,9 +951,11 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
 #pragma unroll
         for (int c = 0; c < NS; c++) {
             const uint32_t pos = (ch0 + (uint32_t)c) * 64 + ln;
-            // the cref kernel writes all-ones for K <= k < Kpad
+            // the cref kernel writes all-ones for K <= k < Kpad; positions of
+            // padding chunks (>= Kpad) all carry id Kpad, whose staged value
+            // is all ones
             const uint32_t v = pos < Kpad ? (uint32_t)cref[(uint64_t)b0 * Kpad + pos] : 0xFFFFu;
-            p[c] = (pos << 16) | v;
+            p[c] = ((pos < Kpad ? pos : Kpad) << 16) | v;
         }
     }
     if constexpr (ABS) {
@@ -993,13 +996,11 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
             uint32_t *st32 = (uint32_t *)stage;
             __syncthreads();  // every wave has read back its payloads
             for (uint32_t q = threadIdx.x; q < Kpad / 2; q += NT) st32[q] = src[q];
+            if (threadIdx.x == 0) st32[Kpad / 2] = ~0u;  // id Kpad: the padding chunks
             __syncthreads();
 #pragma unroll
-            for (int c = 0; c < NS; c++) {
-                // haplotypes at positions >= Kpad (padding chunks) keep all-ones rows
-                const uint32_t id = p[c] >> 16;
-                p[c] = (id << 16) | (id < Kpad ? (uint32_t)stage[id] : 0xFFFFu);
-            }
+            for (int c = 0; c < NS; c++)
+                p[c] = (p[c] & 0xFFFF0000u) | (uint32_t)stage[p[c] >> 16];
         }
         // pass 1: slot c's ballot words go to lane c; zeros of the wave's chunks.
         // Lanes >= NS of olo/ohi are never read: no initial value (asm).
