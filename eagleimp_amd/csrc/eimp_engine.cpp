@@ -34,6 +34,8 @@ uint32_t cref_max_k();
 hipError_t launch_digest_count0(const TgtDev *tg, uint32_t n, hipStream_t s);
 hipError_t launch_cref_rows(const TgtDev *tg, uint32_t t, uint32_t Mp, uint64_t *rows, uint32_t rw,
                             hipStream_t s);
+hipError_t launch_digest_rows(const TgtDev *tg, const uint32_t *d_n, uint32_t nmax, uint64_t *dig,
+                              hipStream_t s);
 int pbwt_variant_count();
 int pbwt_variant_threads(int v);
 int pbwt_variant_cpw(int v);
@@ -434,6 +436,33 @@ int copy_target_out(const TgtDev &T, const TgtDev *d_tg, uint32_t t, uint64_t Na
         HIPCHK(launch_cref_rows(d_tg, t, T.Mp, tmp.as<uint64_t>(), rw, s));
         HIPCHK(hipStreamSynchronize(s));
         HIPCHK(hipMemcpy(cref_rows, tmp.p, (size_t)T.Mp * rw * 8, hipMemcpyDeviceToHost));
+    }
+    return EIMP_OK;
+}
+
+// Mono bits of a target (Target::findMonos, src/Target.cpp:878-897: site m
+// of the window is mono when all chosen haplotypes agree there) from the
+// device's AND / OR words: mono = and | ~or.  Subset targets keep window-
+// relative words (mand_w / mor_w), whole-panel targets read the panel-level
+// words at bit offset s0 (src_words readable words either way).  monos
+// receives roundToMultiple(W,512)/64 words, bits past W zero.
+int copy_monos_out(const TgtDev &T, uint64_t src_words, uint64_t *monos) {
+    if (!T.mand || !T.mor) return fail(EIMP_ERR_STATE, "no mono words for this target");
+    const uint64_t W = T.W, nw = (W + 63) / 64, uw = unit_words(W);
+    const uint64_t q0 = T.mono_off >> 6, sh = T.mono_off & 63;
+    const uint64_t ncopy = std::min<uint64_t>(nw + 1, src_words - q0);
+    std::vector<uint64_t> a(nw + 1, 0), o(nw + 1, 0);
+    HIPCHK(hipMemcpy(a.data(), T.mand + q0, ncopy * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(o.data(), T.mor + q0, ncopy * 8, hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < uw; i++) {
+        uint64_t v = 0;
+        if (i < nw) {
+            const uint64_t av = sh ? (a[i] >> sh) | (a[i + 1] << (64 - sh)) : a[i];
+            const uint64_t ov = sh ? (o[i] >> sh) | (o[i + 1] << (64 - sh)) : o[i];
+            v = av | ~ov;
+            if (i == nw - 1 && (W & 63)) v &= (1ull << (W & 63)) - 1;
+        }
+        monos[i] = v;
     }
     return EIMP_OK;
 }
@@ -1000,6 +1029,15 @@ int eimp_engine_copy_target(eimp_engine *e, size_t t, uint32_t *besthaps, uint64
                            besthaps, bhflags, splitsites, splits, cMdiffs, count0, pbwt, cref_rows);
 }
 
+int eimp_engine_copy_monos(eimp_engine *e, size_t t, uint64_t *monos) {
+    if (!e || !e->initialized) return fail(EIMP_ERR_STATE, "engine not initialized");
+    if (t >= e->last_n || t >= e->h_tg.size()) return fail(EIMP_ERR_INVALID, "target index out of range");
+    if (!monos) return fail(EIMP_ERR_INVALID, "null monos");
+    CHK(set_device(e));
+    const TgtDev &T = e->h_tg[t];
+    return copy_monos_out(T, T.mand == T.mand_w ? unit_words(T.W) : unit_words(e->M), monos);
+}
+
 int eimp_engine_ref_pbwt(eimp_engine *e, size_t m0, size_t nsites, uint32_t *rows, int32_t *count0,
                          int32_t *absperm, double *device_ms) {
     if (!e || !e->initialized) return fail(EIMP_ERR_STATE, "engine not initialized");
@@ -1269,6 +1307,31 @@ int eimp_stream_copy_target(eimp_stream *s, size_t t, uint32_t *besthaps, uint64
     CHK(set_device(s->e));
     return copy_target_out(s->h_tg()[t], s->d_tg(), (uint32_t)t, s->e->Nad, s->stream, s->d_tmp,
                            besthaps, bhflags, splitsites, splits, cMdiffs, count0, pbwt, cref_rows);
+}
+
+int eimp_stream_copy_monos(eimp_stream *s, size_t t, uint64_t *monos) {
+    if (!s) return fail(EIMP_ERR_INVALID, "null stream");
+    if (s->pending || !s->collected) return fail(EIMP_ERR_STATE, "no collected submission");
+    if (t >= s->n_last) return fail(EIMP_ERR_INVALID, "target index out of range");
+    if (!monos) return fail(EIMP_ERR_INVALID, "null monos");
+    CHK(set_device(s->e));
+    const TgtDev &T = s->h_tg()[t];
+    return copy_monos_out(T, T.mand == T.mand_w ? unit_words(T.W) : unit_words(s->e->M), monos);
+}
+
+int eimp_stream_digest_outputs(eimp_stream *s, uint64_t *digests) {
+    if (!s || !digests) return fail(EIMP_ERR_INVALID, "null argument");
+    if (s->pending || !s->collected) return fail(EIMP_ERR_STATE, "no collected submission");
+    if (batch_layout(s->flags) == LAYOUT_RAW)
+        return fail(EIMP_ERR_UNSUPPORTED, "raw rows carry no zero counts to digest");
+    CHK(set_device(s->e));
+    const uint32_t nt = s->n_last;
+    if (!nt) return EIMP_OK;
+    HIPCHK(hipMemsetAsync(s->d_digest.p, 0, (size_t)nt * 16, s->stream));
+    HIPCHK(launch_digest_rows(s->d_tg(), &s->d_batch()->n, nt, s->d_digest.as<uint64_t>(), s->stream));
+    HIPCHK(hipMemcpyAsync(digests, s->d_digest.p, (size_t)nt * 16, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return EIMP_OK;
 }
 
 int eimp_stream_destroy(eimp_stream *s) {

@@ -1473,6 +1473,36 @@ __global__ __launch_bounds__(256) void k_digest_count0(const TgtDev *__restrict_
     if (lane_id() == 0) atomicAdd((unsigned long long *)&T.digest[1], (unsigned long long)s);
 }
 
+// Digests recomputed from stored rows (data / cumulative-zero pairs, the
+// drop-in and compact layouts) and count0: the same sums as the in-kernel
+// digest, for outputs built without it.  grid (DIGR_G, n), block 256: row
+// group x of target y; dig receives 2 words per target (zeroed beforehand).
+constexpr uint32_t DIGR_G = 64;
+__global__ __launch_bounds__(256) void k_digest_rows(const TgtDev *__restrict__ tg,
+                                                    const uint32_t *__restrict__ d_n,
+                                                    uint64_t *__restrict__ dig) {
+    if (blockIdx.y >= *d_n) return;
+    const TgtDev &T = tg[blockIdx.y];
+    if (T.Mp == 0 || !T.out) return;
+    const uint32_t KW = 2 * ((T.K + 31) / 32), RS = T.RS;
+    const uint64_t rows = 2ull * T.Mp;
+    const uint64_t per = (rows + DIGR_G - 1) / DIGR_G;
+    const uint64_t r0 = per * blockIdx.x, r1 = min(rows, r0 + per);
+    uint64_t sp = 0, sc = 0;
+    for (uint64_t r = r0; r < r1; r++) {
+        const uint32_t *row = T.out + r * RS;
+        for (uint32_t w = threadIdx.x; w < KW; w += 256) sp += dig_word(r * KW + w, row[w]);
+    }
+    if (blockIdx.x == 0)
+        for (uint32_t r = threadIdx.x; r < T.Mp; r += 256) sc += dig_count0(r, T.count0[r]);
+    sp = wave_sum64(sp);
+    sc = wave_sum64(sc);
+    if (lane_id() == 0) {
+        atomicAdd((unsigned long long *)&dig[2 * blockIdx.y], (unsigned long long)sp);
+        if (blockIdx.x == 0) atomicAdd((unsigned long long *)&dig[2 * blockIdx.y + 1], (unsigned long long)sc);
+    }
+}
+
 // cref blocks -> condensed reference rows (for inspection).  grid (Mp, 1), block 256
 __global__ __launch_bounds__(256) void k_cref_rows(const TgtDev *__restrict__ tg, uint32_t t,
                                                   uint64_t *__restrict__ rows, uint32_t rw) {
@@ -1581,6 +1611,12 @@ uint32_t cref_max_k() { return 65536 - 64; }
 hipError_t launch_digest_count0(const TgtDev *tg, uint32_t n, hipStream_t s) {
     if (!n) return hipSuccess;
     hipLaunchKernelGGL(k_digest_count0, dim3(n), dim3(256), 0, s, tg, (const uint32_t *)nullptr);
+    return hipGetLastError();
+}
+hipError_t launch_digest_rows(const TgtDev *tg, const uint32_t *d_n, uint32_t nmax, uint64_t *dig,
+                              hipStream_t s) {
+    if (!nmax) return hipSuccess;
+    hipLaunchKernelGGL(k_digest_rows, dim3(DIGR_G, nmax), dim3(256), 0, s, tg, d_n, dig);
     return hipGetLastError();
 }
 hipError_t launch_cref_rows(const TgtDev *tg, uint32_t t, uint32_t Mp, uint64_t *rows, uint32_t rw,

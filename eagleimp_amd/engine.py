@@ -35,7 +35,8 @@ EXPORTS = [
     "eimp_stream_destroy", "eimp_engine_mem_info", "eimp_stream_kernel_ms",
     "eimp_engine_ref_pbwt", "eimp_phase_default_params", "eimp_phase_target", "eimp_phase_batch",
     "eimp_qref_open", "eimp_qref_close", "eimp_qref_metadata", "eimp_qref_variant",
-    "eimp_qref_read_sites",
+    "eimp_qref_read_sites", "eimp_engine_copy_monos", "eimp_stream_copy_monos",
+    "eimp_stream_digest_outputs",
 ]
 
 _P = C.c_void_p
@@ -89,6 +90,9 @@ def lib():
         L.eimp_engine_mem_info.argtypes = [_P, C.POINTER(_SZ), C.POINTER(_SZ)]
         L.eimp_stream_kernel_ms.argtypes = [_P, _P]
         L.eimp_engine_ref_pbwt.argtypes = [_P, _SZ, _SZ, _P, _P, _P, _P]
+        L.eimp_engine_copy_monos.argtypes = [_P, _SZ, _P]
+        L.eimp_stream_copy_monos.argtypes = [_P, _SZ, _P]
+        L.eimp_stream_digest_outputs.argtypes = [_P, _P]
         _LIB = L
     return _LIB
 
@@ -232,10 +236,18 @@ class Engine:
                            K=arr[:, 3].copy(), dig_pbwt=arr[:, 4].copy(),
                            dig_count0=arr[:, 5].copy())
 
-    def copy_target(self, t, cref=False) -> dict:
+    def copy_target(self, t, cref=False, rows=True) -> dict:
+        """Outputs of target t of the last batch; rows=False leaves out the
+        cPBWT rows (the largest part)."""
         S, Mp, _, K = (int(x) for x in self._last[t, :4])
         return _copy_out(lib().eimp_engine_copy_target, self._h, t, S, Mp, K, self._last_W[t],
-                         self._last_layout, self.Nad, cref)
+                         self._last_layout, self.Nad, cref, rows)
+
+    def copy_monos(self, t) -> np.ndarray:
+        """Mono bits of target t of the last batch (window-relative words)."""
+        out = np.zeros(unit_words(self._last_W[t]), np.uint64)
+        _check(lib().eimp_engine_copy_monos(self._h, t, _p(out)))
+        return out
 
     def ref_pbwt(self, m0=0, nsites=None, absperm=True):
         """Panel PBWT for imputation over sites [m0, m0+nsites): (rows, count0,
@@ -292,7 +304,7 @@ def row_stride(K: int, layout: str) -> int:
     return 4 * unit_words(K)
 
 
-def _copy_out(fn, h, t, S, Mp, K, W, layout, Nad, cref) -> dict:
+def _copy_out(fn, h, t, S, Mp, K, W, layout, Nad, cref, rows=True) -> dict:
     rs = row_stride(K, layout)
     out = dict(
         besthaps=np.zeros(K, np.uint32),
@@ -301,14 +313,15 @@ def _copy_out(fn, h, t, S, Mp, K, W, layout, Nad, cref) -> dict:
         splits=np.zeros(unit_words(W), np.uint64),
         cMdiffs=np.zeros(S + 1 if S else 1, np.float32),
         count0=np.zeros(Mp, np.int32),
-        pbwt=np.zeros(2 * Mp * rs, np.uint32),
+        pbwt=np.zeros(2 * Mp * rs if rows else 0, np.uint32),
     )
     cr = np.zeros(Mp * unit_words(K), np.uint64) if cref else None
     _check(fn(h, t, _p(out["besthaps"]), _p(out["bhflags"]), _p(out["splitsites"]),
-              _p(out["splits"]), _p(out["cMdiffs"]), _p(out["count0"]), _p(out["pbwt"]), _p(cr)))
+              _p(out["splits"]), _p(out["cMdiffs"]), _p(out["count0"]),
+              _p(out["pbwt"]) if rows else None, _p(cr)))
     out["splitsites"] = out["splitsites"][:S]
     out["cMdiffs"] = out["cMdiffs"][:S + 1] if S else out["cMdiffs"][:0]
-    out["pbwt"] = out["pbwt"].reshape(2 * Mp, rs)
+    out["pbwt"] = out["pbwt"].reshape(2 * Mp if rows else 0, rs)
     if cref:
         out["cref"] = cr.reshape(Mp, unit_words(K))
     return out
@@ -392,7 +405,19 @@ class Stream:
         _check(lib().eimp_stream_kernel_ms(self._h, _p(ms)))
         return {k: float(ms[i]) for i, k in enumerate(KCLASSES)}
 
-    def copy_target(self, t, cref=False) -> dict:
+    def digest_outputs(self):
+        """(pbwt digests, count0 digests) recomputed from the rows the last
+        collected submission left in the pool (0 for deferred targets)."""
+        d = np.zeros((max(self._n, 1), 2), np.uint64)
+        _check(lib().eimp_stream_digest_outputs(self._h, _p(d)))
+        return d[:self._n, 0].copy(), d[:self._n, 1].copy()
+
+    def copy_monos(self, t) -> np.ndarray:
+        out = np.zeros(unit_words(self._W[t]), np.uint64)
+        _check(lib().eimp_stream_copy_monos(self._h, t, _p(out)))
+        return out
+
+    def copy_target(self, t, cref=False, rows=True) -> dict:
         S, Mp, _, K = (int(x) for x in self._last[t, :4])
         return _copy_out(lib().eimp_stream_copy_target, self._h, t, S, Mp, K, self._W[t],
-                         self.layout, self.eng.Nad, cref)
+                         self.layout, self.eng.Nad, cref, rows)

@@ -153,6 +153,12 @@ int eimp_engine_copy_target(eimp_engine *engine, size_t t, uint32_t *besthaps, u
                             uint32_t *splitsites, uint64_t *splits, float *cMdiffs,
                             int32_t *count0, uint32_t *pbwt, uint64_t *cref_rows);
 
+/* Mono bits of target t of the last batch (Target::findMonos,
+ * src/Target.cpp:878-897: bit m set when all K chosen haplotypes carry the
+ * same allele at window site m), roundToMultiple(W,512)/64 uint64 words, bits
+ * past W zero. */
+int eimp_engine_copy_monos(eimp_engine *engine, size_t t, uint64_t *monos);
+
 /* Panel PBWT for imputation: the reference's PBWT(refT, true)
  * (src/PBWT.cpp:53-71,131-153,207-285, built by src/PBWTImputer.cpp:110,119)
  * over sites [m0, m0+nsites) of the resident panel with all NrefhapsAD
@@ -219,6 +225,17 @@ int eimp_stream_kernel_ms(eimp_stream *stream, double *ms);
 int eimp_stream_copy_target(eimp_stream *stream, size_t t, uint32_t *besthaps, uint64_t *bhflags,
                             uint32_t *splitsites, uint64_t *splits, float *cMdiffs,
                             int32_t *count0, uint32_t *pbwt, uint64_t *cref_rows);
+
+/* As eimp_engine_copy_monos, for target t of the last collected submission. */
+int eimp_stream_copy_monos(eimp_stream *stream, size_t t, uint64_t *monos);
+
+/* Digests of the outputs the last collected submission left in the context's
+ * pool, recomputed from its stored rows and count0: the same two sums as
+ * EIMP_BATCH_DIGEST (results[].digest_pbwt / digest_count0), for contexts
+ * created without it -- a check of those very outputs after the fact.
+ * digests: 2 uint64 per target of the submission, 0 for deferred targets.
+ * The raw layout holds no zero counts: EIMP_ERR_UNSUPPORTED. */
+int eimp_stream_digest_outputs(eimp_stream *stream, uint64_t *digests);
 
 /* Waits for outstanding work and releases the context. */
 int eimp_stream_destroy(eimp_stream *stream);

@@ -166,8 +166,10 @@ def make_target(gt, s0, W, K, cMmaxSplit=0.125):
     return t
 
 
-def process_target(pv: PanelView, gt, s0, W, K, cMmaxSplit=0.125, keep=True) -> dict:
-    """Runs a4..a9 for one target and returns every intermediate as numpy."""
+def process_target(pv: PanelView, gt, s0, W, K, cMmaxSplit=0.125, keep=True,
+                   rows=True) -> dict:
+    """Runs a4..a9 for one target and returns every intermediate as numpy
+    (rows=False: all but the condensed reference and the cPBWT rows)."""
     K = min(int(K), int(pv.N_corr))
     t = make_target(gt, s0, W, K, cMmaxSplit)
     r = EoResult()
@@ -191,10 +193,11 @@ def process_target(pv: PanelView, gt, s0, W, K, cMmaxSplit=0.125, keep=True) -> 
         out["splitsites"] = arr(r.splitsites, r.S, C.c_uint32, np.uint32)
         out["callLocs"] = arr(r.callLocs, r.ncalls, C.c_uint32, np.uint32)
         out["cMdiffs"] = arr(r.cMdiffs, r.S + 1 if r.S else 0, C.c_float, np.float32)
-        out["cref"] = arr(r.cref, r.Mp * Kw, C.c_uint64, np.uint64).reshape(r.Mp, Kw)
         out["count0"] = arr(r.count0, r.Mp, C.c_int32, np.int32)
-        out["pbwt"] = arr(r.pbwt, 2 * r.Mp * pbwt_row_words(K), C.c_uint32,
-                          np.uint32).reshape(2 * r.Mp, pbwt_row_words(K))
+        if rows:
+            out["cref"] = arr(r.cref, r.Mp * Kw, C.c_uint64, np.uint64).reshape(r.Mp, Kw)
+            out["pbwt"] = arr(r.pbwt, 2 * r.Mp * pbwt_row_words(K), C.c_uint32,
+                              np.uint32).reshape(2 * r.Mp, pbwt_row_words(K))
         lib().eo_result_free(C.byref(r))
     return out
 

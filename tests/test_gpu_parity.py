@@ -29,7 +29,8 @@ def _run_and_compare(panel, pv, eng, gts, s0s, Ws, Ks, ignore=None, compact=Fals
         assert int(res.Mp[t]) == r["Mp"], f"{what}: Mp"
         assert int(res.ncalls[t]) == r["ncalls"], f"{what}: ncalls"
         got = eng.copy_target(t, cref=cref)
-        _check_target(r, got, int(r["K"]), "raw" if raw else ("compact" if compact else "gpu"), what)
+        _check_target(r, got, int(r["K"]), "raw" if raw else ("compact" if compact else "gpu"), what,
+                      monos=eng.copy_monos(t))
         assert int(res.dig_pbwt[t]) == r["dig_pbwt"], f"{what}: pbwt digest"
         assert int(res.dig_count0[t]) == r["dig_count0"], f"{what}: count0 digest"
     return res

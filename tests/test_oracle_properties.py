@@ -55,6 +55,55 @@ def test_best_haps_brute_force(data):
         assert np.array_equal(bits(r["flags"], p.N), np.isin(np.arange(p.N), exp).astype(np.uint8))
 
 
+def test_best_haps_haploid_and_ignored_brute_force(data):
+    """findBestHaps with haploid reference samples and an ignored sample
+    (src/Target.cpp:791-853): the candidates are every haplotype except both of
+    the ignored sample and the second of each haploid sample; K >= that count
+    takes all of them in index order, otherwise the K smallest (nerr, index)
+    pairs, reported by index.  Written from that rule, not from the oracle;
+    all-missing targets make every distance equal, so the index tie-break
+    decides among the candidates alone."""
+    p, tg, _ = data
+    H = np.unpackbits(p.hap.view(np.uint8), bitorder="little").reshape(p.N, -1)
+    rng = np.random.default_rng(2024)
+    for trial in range(6):
+        hapl = (rng.random(p.N // 2) < (0.0 if trial == 0 else 0.25)).astype(np.uint8)
+        ignore = -1 if trial == 1 else int(rng.integers(0, p.N // 2))
+        if trial == 2:
+            hapl[ignore] = 1  # an ignored sample that is itself haploid
+        allowed = np.ones(p.N, bool)
+        allowed[1::2][hapl == 1] = False
+        if ignore >= 0:
+            allowed[2 * ignore:2 * ignore + 2] = False
+        cand = np.nonzero(allowed)[0]
+        ncorr = len(cand)
+        pv = oracle.PanelView(p.hap, p.site, p.cM, haploid_ref=hapl, ignore_sample=ignore)
+        assert pv.N_corr == ncorr
+        for t in range(len(tg)):
+            s0, W = int(tg.s0[t]), int(tg.W[t])
+            gt = tg.gt[t]
+            if t == 0:  # all missing: every candidate has nerr = W
+                gt = np.zeros_like(gt)
+                for i in range((W + 63) // 64):
+                    v = (1 << min(64, W - 64 * i)) - 1
+                    gt[2 * i] = gt[2 * i + 1] = np.uint64(v)
+            codes = gt_codes(gt, W)
+            h = H[:, s0:s0 + W]
+            nerr = (((codes == 0) & (h == 1)) | ((codes == 2) & (h == 0)) | (codes == 3)).sum(axis=1)
+            for K in (1, 5, ncorr // 2, ncorr - 1, ncorr, ncorr + 3):
+                Kc = min(K, ncorr)
+                if Kc >= ncorr:
+                    exp = cand
+                else:
+                    order = np.lexsort((cand, nerr[cand]))
+                    exp = np.sort(cand[order[:Kc]])
+                r = oracle.process_target(pv, gt, s0, W, K)
+                what = f"trial {trial} target {t} K {K}"
+                assert np.array_equal(r["besthaps"], exp), what
+                assert np.array_equal(bits(r["flags"], p.N),
+                                      np.isin(np.arange(p.N), exp).astype(np.uint8)), what
+
+
 def test_monos_and_cref_brute_force(data):
     p, tg, pv = data
     H = np.unpackbits(p.hap.view(np.uint8), bitorder="little").reshape(p.N, -1)
