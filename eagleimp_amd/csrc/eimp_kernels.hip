@@ -50,6 +50,9 @@ template <class T>
 __device__ __forceinline__ cptr_t<T> cmem(const T *p) {
     return (cptr_t<T>)p;
 }
+// LDS (address space 3) pointers: 32-bit byte addresses
+template <class T>
+using lds_ptr_t = __attribute__((address_space(3))) T *;
 // compile-time loop: f(std::integral_constant<int, I>) for I in [B, E)
 template <int B, int E, typename F>
 __device__ __forceinline__ void static_for(F &&f) {
@@ -922,6 +925,10 @@ __device__ __forceinline__ uint32_t lane_scan_excl(uint32_t v) {
 // K and windows), 1: one LDS read per lane and a DPP lane scan (fewer
 // instructions and scalar registers; best for uniform targets).  The
 // autotuner picks between the variants built with each.
+//
+// The kernels that call this use dynamic LDS only (the host checks that they
+// have no static LDS), so pay[] starts at LDS address 0 and the scattered
+// stores address it with plain byte offsets.
 template <int NT, int NS, bool ABS = false, int WS = 0>
 __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint32_t *smem,
                                            int32_t *__restrict__ absperm = nullptr,
@@ -933,13 +940,17 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
     const uint32_t nch = Kpad / 64;
     const uint32_t npairs = (K + 31) / 32;
     const uint32_t KW = 2 * npairs;
-    uint32_t *pay = smem;                          // NPOS payloads
+    uint32_t *pay = smem;                          // NPOS payloads, LDS bytes [0, 4*NPOS)
     // Kpad + 2 uint16: the next cref block by haplotype (and all ones at id
     // Kpad), staged at block boundaries (when no payloads are in flight) in
     // the first half of pay[]
     uint16_t *stage = (uint16_t *)smem;
     uint32_t *wzero = smem + NPOS;                 // NW: zeros of each wave's chunks
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = lane_id();
+    // 4 * lane, opaque to the compiler: the uniform-chunk path adds it to a
+    // scalar byte base (one VALU instruction) instead of scaling after the join
+    uint32_t ln4;
+    asm("" : "=v"(ln4) : "0"(ln * 4u));
     const uint32_t ch0 = wave * NS;                // first chunk of this wave
     const gptr_t<const uint16_t> cref = gmem((const uint16_t *)T.cref);
     // wave-uniform flags as scalars: their branches stay scalar branches
@@ -973,6 +984,8 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
     const bool raw = (__builtin_amdgcn_readfirstlane(T.flags) & TF_RAW) != 0;
     const uint32_t och = ch0 + ln;         // chunk stored by this lane
     const bool ostore = ln < (uint32_t)NS && och < nch;
+    // the raw layout has one word pair per chunk, the others two
+    const bool second = !raw && 2 * och + 1 < npairs;
     // digest of this lane's words: g = gbase + step*KW + 4*och (+0..3), key
     // g * kDigKey, rotation -g (alignbit uses 5 bits)
     const uint32_t g0 = (uint32_t)gbase + 4u * och;
@@ -988,10 +1001,31 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
             for (uint32_t w = pw0 / 2; w < RS / 2; w++) prow[w] = u32x2{fw, fw};
         }
     }
+    // this lane's byte offset of its chunk's words within a row (opaque
+    // 32-bit value added to the scalar row address: stores take the
+    // scalar-base form, no 64-bit vector address per step)
+    uint32_t obyte;
+    asm("" : "=v"(obyte) : "0"((raw ? 8u : 16u) * och));
+    const uint32_t rs_bytes = RS * 4u;
+    uint64_t rowaddr = (uint64_t)(size_t)outbase;
+    // row bit of the current row within its 16-row block: forward rows go up
+    // from bit 0, backward rows down from bit 15 (shifts as data, so the step
+    // has no direction branch)
+    uint32_t jbit = 1u << ((rev ? Mp - 1 : 0) & 15);
+    const uint32_t jl = rev ? 0u : 1u, jr = rev ? 1u : 0u;
+    // first step whose row begins a new 16-row block
+    uint32_t nextb = rev ? ((Mp - 1) & 15) + 1 : 16u;
+    // forward pass: count0 of the current 16-row block, lane (step & 15)
+    uint32_t c0acc = 0;
     for (uint32_t step = 0; step < Mp; ++step) {
-        const uint32_t r = rev ? Mp - 1 - step : step;
-        const uint32_t j = r & 15;
-        if (step > 0 && (rev ? j == 15 : j == 0)) {
+        if (step == nextb) {
+            nextb += 16;
+            const uint32_t r = rev ? Mp - 1 - step : step;
+            jbit = rev ? 0x8000u : 1u;
+            if (!rev && wave == 0 && ln < 16u) {  // the previous block's count0
+                if constexpr (ABS) gmem(c0out)[step - 16 + ln] = (int32_t)c0acc;
+                else gmem(T.count0)[step - 16 + ln] = (int32_t)c0acc;
+            }
             const gptr_t<const uint32_t> src = (gptr_t<const uint32_t>)(cref + (uint64_t)(r >> 4) * Kpad);
             uint32_t *st32 = (uint32_t *)stage;
             __syncthreads();  // every wave has read back its payloads
@@ -1002,20 +1036,24 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
             for (int c = 0; c < NS; c++)
                 p[c] = (p[c] & 0xFFFF0000u) | (uint32_t)stage[p[c] >> 16];
         }
-        // pass 1: slot c's ballot words go to lane c; zeros of the wave's chunks.
-        // Lanes >= NS of olo/ohi are never read: no initial value (asm).
-        uint32_t olo, ohi, wz = 0;
+        // pass 1: slot c's ballot words go to lane c.  Lanes >= NS of
+        // olo/ohi are never read: no initial value (asm).
+        uint32_t olo, ohi;
         asm volatile("" : "=v"(olo), "=v"(ohi));
         uint64_t mk[NS];  // the ballots stay in scalar registers for pass 2
         static_for<0, NS>([&](auto ci) {
             constexpr int c = decltype(ci)::value;
-            const uint64_t m = __ballot((p[c] >> j) & 1u);
+            const uint64_t m = __ballot((p[c] & jbit) != 0u);
             mk[c] = m;
             olo = writelane<c>(olo, (uint32_t)m);
             ohi = writelane<c>(ohi, (uint32_t)(m >> 32));
-            wz += 64u - (uint32_t)__popcll(m);
         });
-        if (ln == 0) wzero[wave] = wz;
+        // zeros of lane c's chunk, their exclusive scan over the wave's chunks
+        // (lanes >= NS: don't care) and the wave's total, written to LDS by
+        // every lane (one address, one value)
+        const uint32_t zl = 64u - (uint32_t)__popc(olo) - (uint32_t)__popc(ohi);
+        const uint32_t zex = lane_scan_excl<NS>(zl);
+        wzero[wave] = (uint32_t)__builtin_amdgcn_readlane((int)(zex + zl), NS - 1);
         __syncthreads();
         // zeros before my first chunk, and in the whole row (count0)
         uint32_t run = 0, cnt0 = 0;
@@ -1033,27 +1071,42 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
             run = (uint32_t)__builtin_amdgcn_readlane((int)wex, (int)wave);
             cnt0 = (uint32_t)__builtin_amdgcn_readlane((int)(wex + wv), NW - 1);
         }
-        if constexpr (!ABS) {
-            // count0 of the row: its zeros among the K chosen haplotypes (the
-            // padding positions are all ones); the forward pass writes it
-            if (!rev && threadIdx.x == 0) gmem(T.count0)[r] = (int32_t)cnt0;
-        }
+        // count0 of the row (its zeros among the K chosen haplotypes: the
+        // padding positions are all ones) into lane (step & 15); the forward
+        // pass stores each block's 16 at the next block boundary
+        c0acc = eimp_llvm_writelane(cnt0, step & 15u, c0acc);
         // lane c: zeros before chunk ch0 + c (lanes >= NS: don't care)
-        const uint32_t zbv = run + lane_scan_excl<NS>(64u - (uint32_t)__popc(olo) - (uint32_t)__popc(ohi));
-        // pass 2: stable partition, zeros first.  My bit is bit ln of the
-        // slot's ballot (inverse ballot: the mask itself selects); ones go to
-        // ob + (ones below me), zeros to run + (zeros below me) = run + |ln - t|
-        static_for<0, NS>([&](auto ci) {
-            constexpr int c = decltype(ci)::value;
-            const uint64_t m = mk[c];
-            const uint32_t ob = cnt0 + (ch0 + (uint32_t)c) * 64u - run;
-            const uint32_t t = mbcnt(m);                           // ones below me
-            const uint32_t od = ob + t;
-            const uint32_t zd = __builtin_amdgcn_sad_u16(ln, t, run);
-            const uint32_t dest = __builtin_amdgcn_inverse_ballot_w64(m) ? od : zd;
-            pay[dest] = p[c];
-            run += 64u - (uint32_t)__popcll(m);
-        });
+        const uint32_t zbv = run + zex;
+        // pass 2: stable partition, zeros first.  run / ob: positions of the
+        // zeros / ones of the row before the current chunk (ones go after all
+        // cnt0 zeros).  A chunk whose 64 bits are equal moves as one block
+        // (its lanes in order); otherwise my bit is bit ln of the ballot
+        // (inverse ballot: the mask itself selects), ones go to ob + (ones
+        // below me), zeros to run + (zeros below me) = run + |ln - t|.
+        {
+            uint32_t ob = cnt0 + ch0 * 64u - run;
+            static_for<0, NS>([&](auto ci) {
+                constexpr int c = decltype(ci)::value;
+                const uint64_t m = mk[c];
+                const uint32_t ones = (uint32_t)__popcll(m);
+                uint32_t addr;  // LDS byte address of my destination in pay[]
+                if ((ones & 63u) == 0u) {  // all zeros or all ones: lanes in order
+                    // one vector add of the scalar block base; written as asm so
+                    // that the compiler does not merge it with the other path's
+                    // address arithmetic after the join
+                    const uint32_t base = (ones ? ob : run) * 4u;
+                    asm("v_add_u32 %0, %1, %2" : "=v"(addr) : "s"(base), "v"(ln4));
+                } else {
+                    const uint32_t t = mbcnt(m);  // ones below me
+                    const uint32_t od = ob + t;
+                    const uint32_t zd = __builtin_amdgcn_sad_u16(ln, t, run);
+                    addr = (__builtin_amdgcn_inverse_ballot_w64(m) ? od : zd) * 4u;
+                }
+                *(lds_ptr_t<uint32_t>)(size_t)addr = p[c];
+                run += 64u - ones;
+                ob += ones;
+            });
+        }
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < NS; c++) p[c] = pay[(ch0 + (uint32_t)c) * 64 + ln];
@@ -1066,22 +1119,15 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
                     if (pos < K) ap[pos] = (int32_t)(p[c] >> 16);
                 }
             }
-            if (threadIdx.x == 0) gmem(c0out)[step] = (int32_t)cnt0;
         }
         // the row: chunk ch holds pairs 2ch and 2ch+1 (raw layout: words 2ch, 2ch+1)
-        const gptr_t<uint32_t> row = outbase + (uint64_t)step * RS;
         const uint32_t o1 = zbv + 32u - (uint32_t)__popc(olo);
         const uint32_t o3 = o1 + 32u - (uint32_t)__popc(ohi);
-        const bool second = 2 * och + 1 < npairs;
         // the logical words only: the padding was filled before the loop
         if (ostore) {
-            if (raw) {
-                *(gptr_t<u32x2>)(row + 2 * och) = u32x2{olo, ohi};
-            } else {
-                const gptr_t<u32x2> dst = (gptr_t<u32x2>)(row + 4 * och);
-                dst[0] = u32x2{olo, o1};
-                if (second) dst[1] = u32x2{ohi, o3};
-            }
+            const gptr_t<char> rowb = (gptr_t<char>)(size_t)rowaddr;
+            *(gptr_t<u32x2>)(rowb + obyte) = u32x2{olo, raw ? ohi : o1};
+            if (second) *(gptr_t<u32x2>)(rowb + obyte + 8u) = u32x2{ohi, o3};
         }
         if (dig && ostore) {
             // rotl(v ^ key, g mod 32) == rotr by -g: one v_alignbit each
@@ -1090,11 +1136,21 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
             dsum += __builtin_amdgcn_alignbit(olo ^ dkey, olo ^ dkey, dneg);
             const uint32_t x1 = o1 ^ (dkey + kDigKey);
             dsum += __builtin_amdgcn_alignbit(x1, x1, dneg - 1u);
-            if (second) {
+            if (2 * och + 1 < npairs) {
                 const uint32_t x2 = ohi ^ (dkey + 2u * kDigKey), x3 = o3 ^ (dkey + 3u * kDigKey);
                 dsum += __builtin_amdgcn_alignbit(x2, x2, dneg - 2u);
                 dsum += __builtin_amdgcn_alignbit(x3, x3, dneg - 3u);
             }
+        }
+        rowaddr += rs_bytes;
+        jbit = (jbit << jl) >> jr;
+    }
+    // the forward pass's last (partial) block of count0
+    if (!rev && wave == 0 && Mp > 0) {
+        const uint32_t b0 = (Mp - 1) & ~15u;
+        if (ln < Mp - b0) {
+            if constexpr (ABS) gmem(c0out)[b0 + ln] = (int32_t)c0acc;
+            else gmem(T.count0)[b0 + ln] = (int32_t)c0acc;
         }
     }
     if (dig) {
@@ -1446,13 +1502,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(pbwt_waves_h
 void k_pbwt_persist(const TgtDev *__restrict__ tg,
                                                     BatchDev *__restrict__ B,
                                                     const PbwtItem *__restrict__ items) {
+    // dynamic LDS only, so the payloads start at LDS address 0; the next item's
+    // index passes through smem[0], which no item uses between two items
     extern __shared__ uint32_t smem[];
-    __shared__ uint32_t s_i;
     const uint32_t np = B->npbwt;
     for (;;) {
-        if (threadIdx.x == 0) s_i = atomicAdd(&B->pbwt_ctr, 1u);
+        if (threadIdx.x == 0) smem[0] = atomicAdd(&B->pbwt_ctr, 1u);
         __syncthreads();
-        const uint32_t i = s_i;
+        const uint32_t i = __builtin_amdgcn_readfirstlane(smem[0]);
         __syncthreads();
         if (i >= np) break;
         const PbwtItem it = items[i];
@@ -1533,6 +1590,15 @@ static hipError_t raise_lds_limit(const void *fn, size_t lds) {
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e == hipSuccess) cur = lds;
     return e;
+}
+
+// The PBWT kernels address their payloads from LDS byte 0 (pbwt_steps):
+// they must have no static LDS ahead of the dynamic buffer.
+static hipError_t pbwt_lds_at_zero(const void *fn) {
+    hipFuncAttributes a{};
+    hipError_t e = hipFuncGetAttributes(&a, fn);
+    if (e != hipSuccess) return e;
+    return a.sharedSizeBytes == 0 ? hipSuccess : hipErrorInvalidKernelFile;
 }
 
 hipError_t launch_site_to_hap(const PanelDev &P, uint64_t *hap, hipStream_t s) {
@@ -1655,6 +1721,8 @@ size_t pbwt_lds_bytes(int v, uint32_t Kpad) {
     return npos * 4 + (nt / 64) * 4;      // payloads (the cref stage shares them), wave counts
 }
 hipError_t pbwt_prepare(int v, size_t lds) {
+    hipError_t e = pbwt_lds_at_zero((const void *)kPbwtVariants[v].fn);
+    if (e != hipSuccess) return e;
     return raise_lds_limit((const void *)kPbwtVariants[v].fn, lds);
 }
 hipError_t launch_pbwt(int v, const TgtDev *tg, const PbwtItem *items, uint32_t nitems, size_t lds,
@@ -1682,7 +1750,9 @@ hipError_t launch_panel_pbwt(const PanelDev &P, uint64_t m0, uint32_t nsites, ui
     hipLaunchKernelGGL(k_site_blocks, dim3((nsites + 15) / 16, (nw + 255) / 256), dim3(256), 0, s, P,
                        m0, nsites, blocks, T.Kpad);
     const size_t lds = (size_t)ns * NT * 4 + (NT / 64) * 4;
-    hipError_t e = raise_lds_limit((const void *)k_panel_pbwt<NT, CPW>, lds);
+    hipError_t e = pbwt_lds_at_zero((const void *)k_panel_pbwt<NT, CPW>);
+    if (e != hipSuccess) return e;
+    e = raise_lds_limit((const void *)k_panel_pbwt<NT, CPW>, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_panel_pbwt<NT, CPW>), dim3(1), dim3(NT), lds, s, T, absperm, c0out);
     return hipGetLastError();
@@ -1767,7 +1837,9 @@ hipError_t launch_cref_persist(const PanelDev &P, const TgtDev *tg, BatchDev *B,
 }
 hipError_t pbwt_persist_prepare(int v, size_t lds, int *occ) {
     const void *fn = (const void *)kPbwtVariants[v].fn_persist;
-    hipError_t e = raise_lds_limit(fn, lds);
+    hipError_t e = pbwt_lds_at_zero(fn);
+    if (e != hipSuccess) return e;
+    e = raise_lds_limit(fn, lds);
     if (e != hipSuccess) return e;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, fn, kPbwtVariants[v].nt, lds);
 }
