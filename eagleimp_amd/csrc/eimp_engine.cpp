@@ -11,6 +11,8 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -36,6 +38,9 @@ hipError_t launch_cref_rows(const TgtDev *tg, uint32_t t, uint32_t Mp, uint64_t 
                             hipStream_t s);
 hipError_t launch_digest_rows(const TgtDev *tg, const uint32_t *d_n, uint32_t nmax, uint64_t *dig,
                               hipStream_t s);
+size_t pbwt_big_lds_bytes(uint32_t K);
+uint32_t pbwt_big_max_k();
+hipError_t launch_pbwt_big(const BigItem *items, uint32_t nitems, uint32_t maxK, hipStream_t s);
 int pbwt_variant_count();
 int pbwt_variant_threads(int v);
 int pbwt_variant_cpw(int v);
@@ -168,12 +173,15 @@ struct eimp_engine {
     PanelDev pd{};
     // batch pools
     DevBuf d_tg, d_gt, d_nerr, d_bh, d_bhflags, d_mono, d_splitsites, d_splits, d_cMdiffs,
-        d_stats, d_cref, d_count0, d_out, d_digest, d_tlist, d_citems, d_pitems, d_tmp;
+        d_stats, d_cref, d_count0, d_out, d_digest, d_tlist, d_citems, d_pitems, d_tmp,
+        d_crows, d_ord, d_big;  // large-K PBWT: row-major cref, orders, items
     DevBuf p_blocks, p_rows, p_c0, p_abs;  // panel PBWT
     HostBuf h_stage;
     std::vector<TgtDev> h_tg;
     std::vector<uint32_t> h_stats;
     uint32_t last_n = 0;
+    TgtDev *last_tg = nullptr;    // device target table of the last batch
+    uint64_t *last_dig = nullptr; // its digests
     // timing
     bool timing = false;
     double ms[EIMP_NKCLASS] = {};
@@ -227,7 +235,7 @@ struct eimp_engine {
     void release_pools() {
         DevBuf *bufs[] = {&d_tg, &d_gt, &d_nerr, &d_bh, &d_bhflags, &d_mono, &d_splitsites,
                           &d_splits, &d_cMdiffs, &d_stats, &d_cref, &d_count0, &d_out,
-                          &d_digest, &d_tlist, &d_citems, &d_pitems, &d_tmp,
+                          &d_digest, &d_tlist, &d_citems, &d_pitems, &d_tmp, &d_crows, &d_ord, &d_big,
                           &p_blocks, &p_rows, &p_c0, &p_abs};
         for (auto *b : bufs) b->release();
         h_stage.release();
@@ -300,15 +308,83 @@ int ensure_pmono(eimp_engine *e) {
     return e->sync();
 }
 
+// Where one batch's device buffers come from (the engine's grow-only pools for
+// eimp_engine_run_batch and the drop-in call).
+enum MemSlot : int {
+    MS_TG, MS_NERR, MS_BH, MS_BHF, MS_MONO, MS_SS, MS_SPL, MS_CMD, MS_STATS, MS_TLIST,
+    MS_CREF, MS_DIG, MS_CITEMS, MS_PITEMS, MS_CROWS, MS_ORD, MS_BIG
+};
+struct BatchMem {
+    virtual ~BatchMem() = default;
+    // device buffer of at least `bytes` for a part of the batch
+    virtual int get(MemSlot s, size_t bytes, const char *what, void **p) = 0;
+    // sets T.out and T.count0 of targets 0..n-1 (Mp and RS set)
+    virtual int place_outputs(std::vector<TgtDev> &tg, uint32_t n) = 0;
+    // false: nothing may be allocated (resident panel parts must exist already)
+    virtual bool may_alloc() const { return true; }
+};
+
+struct PoolMem final : BatchMem {
+    eimp_engine *e;
+    explicit PoolMem(eimp_engine *e_) : e(e_) {}
+    DevBuf &slot(MemSlot s) {
+        switch (s) {
+        case MS_TG: return e->d_tg;
+        case MS_NERR: return e->d_nerr;
+        case MS_BH: return e->d_bh;
+        case MS_BHF: return e->d_bhflags;
+        case MS_MONO: return e->d_mono;
+        case MS_SS: return e->d_splitsites;
+        case MS_SPL: return e->d_splits;
+        case MS_CMD: return e->d_cMdiffs;
+        case MS_STATS: return e->d_stats;
+        case MS_TLIST: return e->d_tlist;
+        case MS_CREF: return e->d_cref;
+        case MS_DIG: return e->d_digest;
+        case MS_CITEMS: return e->d_citems;
+        case MS_CROWS: return e->d_crows;
+        case MS_ORD: return e->d_ord;
+        case MS_BIG: return e->d_big;
+        default: return e->d_pitems;
+        }
+    }
+    int get(MemSlot s, size_t bytes, const char *what, void **p) override {
+        DevBuf &b = slot(s);
+        CHK(b.ensure(bytes, what));
+        *p = b.p;
+        return EIMP_OK;
+    }
+    int place_outputs(std::vector<TgtDev> &tg, uint32_t n) override {
+        uint64_t c0_bytes = 0, out_bytes = 0;
+        std::vector<uint64_t> c0_off(n), out_off(n);
+        for (uint32_t t = 0; t < n; t++) {
+            c0_off[t] = c0_bytes;
+            c0_bytes += round_up((uint64_t)tg[t].Mp * 4, 256);
+            out_off[t] = out_bytes;
+            out_bytes += round_up(2ull * tg[t].Mp * tg[t].RS * 4, 256);
+        }
+        CHK(e->d_count0.ensure(c0_bytes, "count0"));
+        CHK(e->d_out.ensure(out_bytes, "cPBWT output"));
+        for (uint32_t t = 0; t < n; t++) {
+            tg[t].count0 = reinterpret_cast<int32_t *>(e->d_count0.as<char>() + c0_off[t]);
+            tg[t].out = reinterpret_cast<uint32_t *>(e->d_out.as<char>() + out_off[t]);
+        }
+        return EIMP_OK;
+    }
+};
+
 // Phase B for targets whose phase-A fields (bh, splitsites, S) are in place.
 // Sizes outputs, builds cref + PBWT in the given output layout (LAYOUT_*).
-int run_build(eimp_engine *e, uint32_t n, bool digest, uint32_t layout) {
+int run_build(eimp_engine *e, uint32_t n, bool digest, uint32_t layout, BatchMem &mem) {
     std::vector<TgtDev> &tg = e->h_tg;
-    uint64_t cref_bytes = 0, c0_bytes = 0, out_bytes = 0;
-    std::vector<uint64_t> cref_off(n), c0_off(n), out_off(n);
-    uint32_t Kmax_pad = 0, Kmax = 0;
+    uint64_t cref_bytes = 0, crows_bytes = 0, ord_bytes = 0;
+    std::vector<uint64_t> cref_off(n), crows_off(n, 0), ord_off(n, 0);
+    uint32_t Kmax_pad = 0, Kmax = 0, Kbig = 0;
     std::vector<CrefItem> citems, citems_id;  // general / identity targets
     std::vector<PbwtItem> pitems;
+    std::vector<uint32_t> big;                // targets beyond the LDS-resident PBWT kernels
+    uint32_t lds_cap = 0;
+    for (int v = 0; v < pbwt_variant_count(); v++) lds_cap = std::max(lds_cap, pbwt_variant_maxk(v));
     for (uint32_t t = 0; t < n; t++) {
         TgtDev &T = tg[t];
         T.Mp = 2 * T.S + 1;
@@ -318,11 +394,6 @@ int run_build(eimp_engine *e, uint32_t n, bool digest, uint32_t layout) {
         const uint64_t nblk = (T.Mp + 15) / 16;
         cref_off[t] = cref_bytes;
         cref_bytes += round_up(nblk * T.Kpad * 2, 256);
-        c0_off[t] = c0_bytes;
-        c0_bytes += round_up((uint64_t)T.Mp * 4, 256);
-        out_off[t] = out_bytes;
-        out_bytes += round_up(2ull * T.Mp * T.RS * 4, 256);
-        Kmax_pad = std::max(Kmax_pad, T.Kpad);
         const bool ident = (T.flags & TF_IDENTITY) && e->Nad <= 2ull * 256 * 64;
         if (!ident) Kmax = std::max(Kmax, T.K);
         if (ident) {
@@ -331,62 +402,132 @@ int run_build(eimp_engine *e, uint32_t n, bool digest, uint32_t layout) {
             for (uint32_t b = 0; b < nblk; b += kCrefGroup)
                 citems.push_back(CrefItem{t, b, (uint32_t)std::min<uint64_t>(nblk, b + kCrefGroup), 0});
         }
-        pitems.push_back(PbwtItem{t, 0});
-        pitems.push_back(PbwtItem{t, 1});
+        if (T.Kpad > lds_cap) {
+            // order in global memory: rows of K bits, two orders of Kpad per direction
+            if (T.K > pbwt_big_max_k())
+                return fail(EIMP_ERR_UNSUPPORTED, "target " + std::to_string(t) + ": K beyond " +
+                                                      std::to_string(pbwt_big_max_k()));
+            big.push_back(t);
+            Kbig = std::max(Kbig, T.K);
+            crows_off[t] = crows_bytes;
+            crows_bytes += round_up((uint64_t)T.Mp * unit_words(T.K) * 8, 256);
+            ord_off[t] = ord_bytes;
+            ord_bytes += 4ull * T.Kpad * 4;
+        } else {
+            Kmax_pad = std::max(Kmax_pad, T.Kpad);
+            pitems.push_back(PbwtItem{t, 0});
+            pitems.push_back(PbwtItem{t, 1});
+        }
     }
-    if (Kmax > cref_max_k()) return fail(EIMP_ERR_UNSUPPORTED, "K beyond the cref kernel limit");
     // the subset cref kernel reads the haplotype-major panel (the drop-in's
     // initialize uploads only the site-major one: transposed here, once)
-    if (!citems.empty()) CHK(ensure_hap(e));
+    if (!citems.empty() && !e->have_hap) {
+        if (!mem.may_alloc())
+            return fail(EIMP_ERR_STATE, "the panel's hap-major rows are not resident");
+        CHK(ensure_hap(e));
+    }
     if (cref_lds_bytes(e->pd, Kmax) > 160 * 1024)
         return fail(EIMP_ERR_UNSUPPORTED, "panel too wide for the cref kernel's LDS staging");
-    int v = e->choose_pbwt_variant(Kmax_pad);
-    if (v < 0) return v;
-    const size_t lds = pbwt_lds_bytes(v, Kmax_pad);
-    if (lds > 160 * 1024)
-        return fail(EIMP_ERR_UNSUPPORTED, "K beyond the PBWT kernel's slots or 160 KiB of LDS");
-    if (e->pbwt_prepared_variant != v || e->pbwt_prepared_lds < lds) {
-        HIPCHK(pbwt_prepare(v, lds));
-        e->pbwt_prepared_variant = v;
-        e->pbwt_prepared_lds = lds;
+    int v = -1;
+    size_t lds = 0;
+    if (!pitems.empty()) {
+        v = e->choose_pbwt_variant(Kmax_pad);
+        if (v < 0) return v;
+        lds = pbwt_lds_bytes(v, Kmax_pad);
+        if (lds > 160 * 1024)
+            return fail(EIMP_ERR_UNSUPPORTED, "K beyond the PBWT kernel's slots or 160 KiB of LDS");
+        if (e->pbwt_prepared_variant != v || e->pbwt_prepared_lds < lds) {
+            HIPCHK(pbwt_prepare(v, lds));
+            e->pbwt_prepared_variant = v;
+            e->pbwt_prepared_lds = lds;
+        }
     }
+    if (!big.empty() && pbwt_big_lds_bytes(Kbig) > 160 * 1024)
+        return fail(EIMP_ERR_UNSUPPORTED, "K beyond the large-K PBWT kernel's LDS");
     // longest targets first
     std::stable_sort(pitems.begin(), pitems.end(), [&](const PbwtItem &a, const PbwtItem &b) {
         return tg[a.t].Mp > tg[b.t].Mp;
     });
-    CHK(e->d_cref.ensure(cref_bytes, "cref"));
-    CHK(e->d_count0.ensure(c0_bytes, "count0"));
-    CHK(e->d_out.ensure(out_bytes, "cPBWT output"));
-    CHK(e->d_digest.ensure((size_t)n * 16, "digest"));
+    void *p_cref = nullptr, *p_dig = nullptr, *p_ci = nullptr, *p_pi = nullptr, *p_tg = nullptr;
+    void *p_crows = nullptr, *p_ord = nullptr, *p_big = nullptr;
+    CHK(mem.get(MS_CREF, cref_bytes, "cref", &p_cref));
+    CHK(mem.get(MS_DIG, (size_t)n * 16, "digest", &p_dig));
     const size_t nci = citems.size();
     citems.insert(citems.end(), citems_id.begin(), citems_id.end());
-    CHK(e->d_citems.ensure(std::max<size_t>(citems.size(), 1) * sizeof(CrefItem), "cref items"));
-    CHK(e->d_pitems.ensure(pitems.size() * sizeof(PbwtItem), "pbwt items"));
+    CHK(mem.get(MS_CITEMS, std::max<size_t>(citems.size(), 1) * sizeof(CrefItem), "cref items", &p_ci));
+    CHK(mem.get(MS_PITEMS, std::max<size_t>(pitems.size(), 1) * sizeof(PbwtItem), "pbwt items", &p_pi));
+    if (!big.empty()) {
+        CHK(mem.get(MS_CROWS, crows_bytes, "large-K cref rows", &p_crows));
+        CHK(mem.get(MS_ORD, ord_bytes, "large-K orders", &p_ord));
+        CHK(mem.get(MS_BIG, 2 * big.size() * sizeof(BigItem), "large-K items", &p_big));
+    }
+    CHK(mem.get(MS_TG, n * sizeof(TgtDev), "target table", &p_tg));
+    CHK(mem.place_outputs(tg, n));
+    uint64_t *d_dig = static_cast<uint64_t *>(p_dig);
+    TgtDev *d_tg = static_cast<TgtDev *>(p_tg);
+    CrefItem *d_ci = static_cast<CrefItem *>(p_ci);
+    PbwtItem *d_pi = static_cast<PbwtItem *>(p_pi);
     for (uint32_t t = 0; t < n; t++) {
         TgtDev &T = tg[t];
-        T.cref = reinterpret_cast<uint16_t *>(e->d_cref.as<char>() + cref_off[t]);
-        T.count0 = reinterpret_cast<int32_t *>(e->d_count0.as<char>() + c0_off[t]);
-        T.out = reinterpret_cast<uint32_t *>(e->d_out.as<char>() + out_off[t]);
-        T.digest = digest ? e->d_digest.as<uint64_t>() + 2 * t : nullptr;
+        T.cref = reinterpret_cast<uint16_t *>(static_cast<char *>(p_cref) + cref_off[t]);
+        T.digest = digest ? d_dig + 2 * t : nullptr;
     }
-    if (digest) HIPCHK(hipMemsetAsync(e->d_digest.p, 0, (size_t)n * 16, e->stream));
-    HIPCHK(hipMemcpyAsync(e->d_tg.p, tg.data(), n * sizeof(TgtDev), hipMemcpyHostToDevice, e->stream));
+    std::vector<BigItem> bitems;
+    for (uint32_t t : big) {
+        const TgtDev &T = tg[t];
+        const uint64_t *rows = reinterpret_cast<const uint64_t *>(static_cast<char *>(p_crows) + crows_off[t]);
+        uint32_t *ord = reinterpret_cast<uint32_t *>(static_cast<char *>(p_ord) + ord_off[t]);
+        const uint32_t KW = 2 * ((T.K + 31) / 32);
+        for (uint32_t dir = 0; dir < 2; dir++) {
+            BigItem B{};
+            B.rows = rows;
+            B.rstride = unit_words(T.K);
+            B.out = T.out + (dir ? (uint64_t)T.Mp * T.RS : 0ull);
+            B.count0 = dir ? nullptr : T.count0;
+            B.absperm = nullptr;
+            B.ord0 = ord + (uint64_t)dir * 2 * T.Kpad;
+            B.ord1 = B.ord0 + T.Kpad;
+            B.digest = T.digest;
+            B.gbase = dir ? (uint64_t)T.Mp * KW : 0ull;
+            B.K = T.K;
+            B.Mp = T.Mp;
+            B.RS = T.RS;
+            B.rev = dir;
+            B.raw = layout == LAYOUT_RAW ? 1u : 0u;
+            bitems.push_back(B);
+        }
+    }
+    if (digest) HIPCHK(hipMemsetAsync(d_dig, 0, (size_t)n * 16, e->stream));
+    HIPCHK(hipMemcpyAsync(d_tg, tg.data(), n * sizeof(TgtDev), hipMemcpyHostToDevice, e->stream));
     if (!citems.empty())
-        HIPCHK(hipMemcpyAsync(e->d_citems.p, citems.data(), citems.size() * sizeof(CrefItem),
+        HIPCHK(hipMemcpyAsync(d_ci, citems.data(), citems.size() * sizeof(CrefItem),
                               hipMemcpyHostToDevice, e->stream));
-    HIPCHK(hipMemcpyAsync(e->d_pitems.p, pitems.data(), pitems.size() * sizeof(PbwtItem),
-                          hipMemcpyHostToDevice, e->stream));
+    if (!pitems.empty())
+        HIPCHK(hipMemcpyAsync(d_pi, pitems.data(), pitems.size() * sizeof(PbwtItem),
+                              hipMemcpyHostToDevice, e->stream));
+    if (!bitems.empty())
+        HIPCHK(hipMemcpyAsync(p_big, bitems.data(), bitems.size() * sizeof(BigItem),
+                              hipMemcpyHostToDevice, e->stream));
     int ti = e->tbegin(EIMP_KCLASS_CREF);
-    HIPCHK(launch_cref(e->pd, e->d_tg.as<TgtDev>(), e->d_citems.as<CrefItem>(), (uint32_t)nci,
-                       Kmax, false, e->stream));
-    HIPCHK(launch_cref(e->pd, e->d_tg.as<TgtDev>(), e->d_citems.as<CrefItem>() + nci,
-                       (uint32_t)(citems.size() - nci), 0, true, e->stream));
+    HIPCHK(launch_cref(e->pd, d_tg, d_ci, (uint32_t)nci, Kmax, false, e->stream));
+    HIPCHK(launch_cref(e->pd, d_tg, d_ci + nci, (uint32_t)(citems.size() - nci), 0, true, e->stream));
+    // large-K targets read their condensed reference as rows of K bits
+    for (uint32_t t : big) {
+        const TgtDev &T = tg[t];
+        uint64_t *rows = reinterpret_cast<uint64_t *>(static_cast<char *>(p_crows) + crows_off[t]);
+        HIPCHK(launch_cref_rows(d_tg, t, T.Mp, rows, (uint32_t)unit_words(T.K), e->stream));
+    }
     e->tend(ti);
     ti = e->tbegin(EIMP_KCLASS_PBWT);
-    HIPCHK(launch_pbwt(v, e->d_tg.as<TgtDev>(), e->d_pitems.as<PbwtItem>(), (uint32_t)pitems.size(),
-                       lds, e->stream));
+    if (!pitems.empty())
+        HIPCHK(launch_pbwt(v, d_tg, d_pi, (uint32_t)pitems.size(), lds, e->stream));
+    if (!bitems.empty())
+        HIPCHK(launch_pbwt_big(static_cast<const BigItem *>(p_big), (uint32_t)bitems.size(), Kbig,
+                               e->stream));
     e->tend(ti);
-    if (digest) HIPCHK(launch_digest_count0(e->d_tg.as<TgtDev>(), n, e->stream));
+    if (digest) HIPCHK(launch_digest_count0(d_tg, n, e->stream));
+    e->last_tg = d_tg;
+    e->last_dig = d_dig;
     return EIMP_OK;
 }
 
@@ -464,6 +605,160 @@ int copy_monos_out(const TgtDev &T, uint64_t src_words, uint64_t *monos) {
         }
         monos[i] = v;
     }
+    return EIMP_OK;
+}
+
+// The whole pre-stage for n targets (eimp_engine_run_batch and its
+// no-allocation variant): phase A (distances, top-K, monos, split sites), one
+// read of the split counts, phase B (run_build).  dev_gt: targets[].gt are
+// device pointers (used in place), else host words staged through pinned
+// memory.
+int run_batch_impl(eimp_engine *e, size_t n, const eimp_target *targets, float cMmaxSplit,
+                   uint32_t flags, eimp_target_result *results, BatchMem &mem, bool dev_gt) {
+    if (!targets || !results) return fail(EIMP_ERR_INVALID, "null targets/results");
+    if (!e->have_cM) return fail(EIMP_ERR_STATE, "genetic positions not set (eimp_engine_set_panel_extras)");
+    if (n > (1u << 24)) return fail(EIMP_ERR_UNSUPPORTED, "too many targets in one batch");
+    if ((flags & EIMP_BATCH_COMPACT) && (flags & EIMP_BATCH_RAW))
+        return fail(EIMP_ERR_INVALID, "EIMP_BATCH_COMPACT and EIMP_BATCH_RAW exclude each other");
+    const uint32_t nt = (uint32_t)n;
+    const uint64_t Nw = unit_words(e->Nad);
+    // ---- sizes and validation
+    std::vector<uint64_t> gt_off(nt), ww_off(nt), ss_off(nt), nerr_off(nt), bh_off(nt), bhf_off(nt),
+        mono_off(nt);
+    uint64_t gt_words = 0, w_words = 0, ss_elems = 0, nerr_elems = 0, bh_elems = 0, mono_words = 0;
+    bool need_hap = false, need_pmono = false;
+    std::vector<uint32_t> sel_list, mono_list;
+    uint32_t max_ww = 0;
+    e->h_tg.assign(nt, TgtDev{});
+    for (uint32_t t = 0; t < nt; t++) {
+        const eimp_target &x = targets[t];
+        TgtDev &T = e->h_tg[t];
+        CHK(target_inputs(e, x, t, cMmaxSplit, T));
+        const bool select = (T.flags & TF_SELECT) != 0;
+        const bool ident = (T.flags & TF_IDENTITY) != 0;
+        gt_off[t] = gt_words;
+        gt_words += 2 * unit_words(x.W);
+        ww_off[t] = w_words;
+        w_words += unit_words(x.W);
+        ss_off[t] = ss_elems;
+        ss_elems += round_up(x.W + 1, 64);
+        nerr_off[t] = nerr_elems;
+        if (select) {
+            nerr_elems += round_up(e->Nad, 64);
+            sel_list.push_back(t);
+            need_hap = true;
+        }
+        bh_off[t] = bh_elems;
+        bh_elems += round_up(T.K, 64);
+        bhf_off[t] = (uint64_t)t * Nw;
+        mono_off[t] = mono_words;
+        if (ident) {
+            need_pmono = true;
+        } else {
+            mono_words += unit_words(x.W);
+            mono_list.push_back(t);
+            need_hap = true;
+            max_ww = std::max<uint32_t>(max_ww, (uint32_t)((x.W + 63) / 64));
+        }
+    }
+    if (!mem.may_alloc() && ((need_hap && !e->have_hap) || (need_pmono && !e->have_pmono)))
+        return fail(EIMP_ERR_STATE, "the panel's hap-major rows / mono bits are not resident: pass "
+                                    "hapData to eimp_engine_set_panel_extras (or run one "
+                                    "eimp_engine_run_batch with such targets) first");
+    if (need_hap) CHK(ensure_hap(e));
+    if (need_pmono) CHK(ensure_pmono(e));
+    // ---- device buffers
+    void *p_tg, *p_nerr, *p_bh, *p_bhf, *p_mono, *p_ss, *p_spl, *p_cmd, *p_stats, *p_tl;
+    CHK(mem.get(MS_TG, nt * sizeof(TgtDev), "target table", &p_tg));
+    CHK(mem.get(MS_NERR, std::max<uint64_t>(nerr_elems, 1) * 4, "distances", &p_nerr));
+    CHK(mem.get(MS_BH, bh_elems * 4, "best haps", &p_bh));
+    CHK(mem.get(MS_BHF, (uint64_t)nt * Nw * 8, "best-hap flags", &p_bhf));
+    CHK(mem.get(MS_MONO, std::max<uint64_t>(mono_words, 1) * 16, "monos", &p_mono));
+    CHK(mem.get(MS_SS, ss_elems * 4, "split sites", &p_ss));
+    CHK(mem.get(MS_SPL, w_words * 8, "split bitmap", &p_spl));
+    CHK(mem.get(MS_CMD, ss_elems * 4, "cM diffs", &p_cmd));
+    CHK(mem.get(MS_STATS, (uint64_t)nt * 16, "stats", &p_stats));
+    CHK(mem.get(MS_TLIST, (uint64_t)(sel_list.size() + mono_list.size() + 2) * 4, "target lists", &p_tl));
+    TgtDev *d_tg = static_cast<TgtDev *>(p_tg);
+    // ---- genotypes: in place (device) or one copy from pinned staging
+    const uint64_t *d_gt = nullptr;
+    if (!dev_gt) {
+        CHK(e->d_gt.ensure(gt_words * 8, "genotypes"));
+        CHK(e->h_stage.ensure(gt_words * 8));
+        for (uint32_t t = 0; t < nt; t++) {
+            const uint64_t w = 2 * unit_words(targets[t].W);
+            std::memcpy(e->h_stage.as<uint64_t>() + gt_off[t], targets[t].gt, w * 8);
+        }
+        HIPCHK(hipMemcpyAsync(e->d_gt.p, e->h_stage.p, gt_words * 8, hipMemcpyHostToDevice, e->stream));
+        d_gt = e->d_gt.as<uint64_t>();
+    }
+    uint64_t *mono_base = static_cast<uint64_t *>(p_mono);
+    const uint64_t mono_total = std::max<uint64_t>(mono_words, 1);
+    for (uint32_t t = 0; t < nt; t++) {
+        TgtDev &T = e->h_tg[t];
+        T.gt = dev_gt ? targets[t].gt : d_gt + gt_off[t];
+        T.nerr = static_cast<uint32_t *>(p_nerr) + nerr_off[t];
+        T.bh = static_cast<uint32_t *>(p_bh) + bh_off[t];
+        T.bhflags = static_cast<uint64_t *>(p_bhf) + bhf_off[t];
+        if (T.flags & TF_IDENTITY) {
+            T.mand = e->pand.as<uint64_t>();
+            T.mor = e->por.as<uint64_t>();
+            T.mand_w = nullptr;
+            T.mor_w = nullptr;
+            T.mono_off = T.s0;
+        } else {
+            T.mand_w = mono_base + mono_off[t];
+            T.mor_w = mono_base + mono_total + mono_off[t];
+            T.mand = T.mand_w;
+            T.mor = T.mor_w;
+            T.mono_off = 0;
+        }
+        T.splitsites = static_cast<uint32_t *>(p_ss) + ss_off[t];
+        T.splits = static_cast<uint64_t *>(p_spl) + ww_off[t];
+        T.cMdiffs = static_cast<float *>(p_cmd) + ss_off[t];
+        T.stats = static_cast<uint32_t *>(p_stats) + 4 * t;
+    }
+    std::vector<uint32_t> lists(sel_list);
+    lists.insert(lists.end(), mono_list.begin(), mono_list.end());
+    if (!lists.empty())
+        HIPCHK(hipMemcpyAsync(p_tl, lists.data(), lists.size() * 4, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(d_tg, e->h_tg.data(), nt * sizeof(TgtDev), hipMemcpyHostToDevice, e->stream));
+    const uint32_t *d_sel = static_cast<const uint32_t *>(p_tl);
+    const uint32_t *d_mono_list = d_sel + sel_list.size();
+    // ---- phase A
+    int ti = e->tbegin(EIMP_KCLASS_DISTANCE);
+    HIPCHK(launch_distance(e->pd, d_tg, d_sel, (uint32_t)sel_list.size(), e->stream));
+    e->tend(ti);
+    ti = e->tbegin(EIMP_KCLASS_SELECT);
+    HIPCHK(launch_select(e->pd, d_tg, nt, e->stream));
+    e->tend(ti);
+    ti = e->tbegin(EIMP_KCLASS_MONOS);
+    HIPCHK(launch_monos(e->pd, d_tg, d_mono_list, (uint32_t)mono_list.size(), max_ww, e->stream));
+    e->tend(ti);
+    ti = e->tbegin(EIMP_KCLASS_SPLITS);
+    HIPCHK(launch_splits(e->pd, d_tg, nt, e->stream));
+    e->tend(ti);
+    e->h_stats.resize(4 * (size_t)nt);
+    HIPCHK(hipMemcpyAsync(e->h_stats.data(), p_stats, (size_t)nt * 16, hipMemcpyDeviceToHost, e->stream));
+    CHK(e->sync());
+    for (uint32_t t = 0; t < nt; t++) e->h_tg[t].S = e->h_stats[4 * t];
+    // ---- phase B
+    const bool digest = (flags & EIMP_BATCH_DIGEST) != 0;
+    CHK(run_build(e, nt, digest, batch_layout(flags), mem));
+    std::vector<uint64_t> dig(digest ? 2 * (size_t)nt : 0);
+    if (digest)
+        HIPCHK(hipMemcpyAsync(dig.data(), e->last_dig, (size_t)nt * 16, hipMemcpyDeviceToHost, e->stream));
+    CHK(e->sync());
+    for (uint32_t t = 0; t < nt; t++) {
+        const TgtDev &T = e->h_tg[t];
+        results[t].S = T.S;
+        results[t].Mp = T.Mp;
+        results[t].ncalls = e->h_stats[4 * t + 1];
+        results[t].K = T.K;
+        results[t].digest_pbwt = digest ? dig[2 * t] : 0;
+        results[t].digest_count0 = digest ? dig[2 * t + 1] : 0;
+    }
+    e->last_n = nt;
     return EIMP_OK;
 }
 
@@ -787,154 +1082,21 @@ int eimp_engine_set_panel_extras(eimp_engine *e, const float *cM, const uint64_t
         HIPCHK(hipMemcpy(e->haploid.p, haploidRef, ns, hipMemcpyHostToDevice));
         e->pd.haploid = e->haploid.as<uint8_t>();
     }
+    // per-site all-ones / any-one bits of the whole panel (monos of
+    // whole-panel targets), built here so that later batches need no allocation
+    CHK(ensure_pmono(e));
     return EIMP_OK;
 }
+
+
 
 int eimp_engine_run_batch(eimp_engine *e, size_t n, const eimp_target *targets, float cMmaxSplit,
                           uint32_t flags, eimp_target_result *results) {
     if (!e || !e->initialized) return fail(EIMP_ERR_STATE, "engine not initialized");
     if (n == 0) return EIMP_OK;
-    if (!targets || !results) return fail(EIMP_ERR_INVALID, "null targets/results");
-    if (!e->have_cM) return fail(EIMP_ERR_STATE, "genetic positions not set (eimp_engine_set_panel_extras)");
-    if (n > (1u << 24)) return fail(EIMP_ERR_UNSUPPORTED, "too many targets in one batch");
-    if ((flags & EIMP_BATCH_COMPACT) && (flags & EIMP_BATCH_RAW))
-        return fail(EIMP_ERR_INVALID, "EIMP_BATCH_COMPACT and EIMP_BATCH_RAW exclude each other");
     CHK(set_device(e));
-    const uint32_t nt = (uint32_t)n;
-    const uint64_t Nw = unit_words(e->Nad);
-    // ---- sizes and validation
-    std::vector<uint64_t> gt_off(nt), ww_off(nt), ss_off(nt), nerr_off(nt), bh_off(nt), bhf_off(nt),
-        mono_off(nt);
-    uint64_t gt_words = 0, w_words = 0, ss_elems = 0, nerr_elems = 0, bh_elems = 0, mono_words = 0;
-    bool need_hap = false, need_pmono = false;
-    std::vector<uint32_t> sel_list, mono_list;
-    uint32_t max_ww = 0;
-    e->h_tg.assign(nt, TgtDev{});
-    for (uint32_t t = 0; t < nt; t++) {
-        const eimp_target &x = targets[t];
-        TgtDev &T = e->h_tg[t];
-        CHK(target_inputs(e, x, t, cMmaxSplit, T));
-        const bool select = (T.flags & TF_SELECT) != 0;
-        const bool ident = (T.flags & TF_IDENTITY) != 0;
-        gt_off[t] = gt_words;
-        gt_words += 2 * unit_words(x.W);
-        ww_off[t] = w_words;
-        w_words += unit_words(x.W);
-        ss_off[t] = ss_elems;
-        ss_elems += round_up(x.W + 1, 64);
-        nerr_off[t] = nerr_elems;
-        if (select) {
-            nerr_elems += round_up(e->Nad, 64);
-            sel_list.push_back(t);
-            need_hap = true;
-        }
-        bh_off[t] = bh_elems;
-        bh_elems += round_up(T.K, 64);
-        bhf_off[t] = (uint64_t)t * Nw;
-        mono_off[t] = mono_words;
-        if (ident) {
-            need_pmono = true;
-        } else {
-            mono_words += unit_words(x.W);
-            mono_list.push_back(t);
-            need_hap = true;
-            max_ww = std::max<uint32_t>(max_ww, (uint32_t)((x.W + 63) / 64));
-        }
-    }
-    if (need_hap) CHK(ensure_hap(e));
-    if (need_pmono) CHK(ensure_pmono(e));
-    // ---- pools
-    CHK(e->d_tg.ensure(nt * sizeof(TgtDev), "target table"));
-    CHK(e->d_gt.ensure(gt_words * 8, "genotypes"));
-    CHK(e->d_nerr.ensure(std::max<uint64_t>(nerr_elems, 1) * 4, "distances"));
-    CHK(e->d_bh.ensure(bh_elems * 4, "best haps"));
-    CHK(e->d_bhflags.ensure((uint64_t)nt * Nw * 8, "best-hap flags"));
-    CHK(e->d_mono.ensure(std::max<uint64_t>(mono_words, 1) * 16, "monos"));
-    CHK(e->d_splitsites.ensure(ss_elems * 4, "split sites"));
-    CHK(e->d_splits.ensure(w_words * 8, "split bitmap"));
-    CHK(e->d_cMdiffs.ensure(ss_elems * 4, "cM diffs"));
-    CHK(e->d_stats.ensure((uint64_t)nt * 16, "stats"));
-    CHK(e->d_tlist.ensure((uint64_t)(sel_list.size() + mono_list.size() + 2) * 4, "target lists"));
-    CHK(e->h_stage.ensure(gt_words * 8));
-    // ---- genotypes to the device (one copy from pinned staging)
-    for (uint32_t t = 0; t < nt; t++) {
-        const uint64_t w = 2 * unit_words(targets[t].W);
-        std::memcpy(e->h_stage.as<uint64_t>() + gt_off[t], targets[t].gt, w * 8);
-    }
-    HIPCHK(hipMemcpyAsync(e->d_gt.p, e->h_stage.p, gt_words * 8, hipMemcpyHostToDevice, e->stream));
-    uint64_t *mono_base = e->d_mono.as<uint64_t>();
-    const uint64_t mono_total = std::max<uint64_t>(mono_words, 1);
-    for (uint32_t t = 0; t < nt; t++) {
-        TgtDev &T = e->h_tg[t];
-        T.gt = e->d_gt.as<uint64_t>() + gt_off[t];
-        T.nerr = e->d_nerr.as<uint32_t>() + nerr_off[t];
-        T.bh = e->d_bh.as<uint32_t>() + bh_off[t];
-        T.bhflags = e->d_bhflags.as<uint64_t>() + bhf_off[t];
-        if (T.flags & TF_IDENTITY) {
-            T.mand = e->pand.as<uint64_t>();
-            T.mor = e->por.as<uint64_t>();
-            T.mand_w = nullptr;
-            T.mor_w = nullptr;
-            T.mono_off = T.s0;
-        } else {
-            T.mand_w = mono_base + mono_off[t];
-            T.mor_w = mono_base + mono_total + mono_off[t];
-            T.mand = T.mand_w;
-            T.mor = T.mor_w;
-            T.mono_off = 0;
-        }
-        T.splitsites = e->d_splitsites.as<uint32_t>() + ss_off[t];
-        T.splits = e->d_splits.as<uint64_t>() + ww_off[t];
-        T.cMdiffs = e->d_cMdiffs.as<float>() + ss_off[t];
-        T.stats = e->d_stats.as<uint32_t>() + 4 * t;
-    }
-    std::vector<uint32_t> lists(sel_list);
-    lists.insert(lists.end(), mono_list.begin(), mono_list.end());
-    if (!lists.empty())
-        HIPCHK(hipMemcpyAsync(e->d_tlist.p, lists.data(), lists.size() * 4, hipMemcpyHostToDevice,
-                              e->stream));
-    HIPCHK(hipMemcpyAsync(e->d_tg.p, e->h_tg.data(), nt * sizeof(TgtDev), hipMemcpyHostToDevice,
-                          e->stream));
-    const uint32_t *d_sel = e->d_tlist.as<uint32_t>();
-    const uint32_t *d_mono_list = d_sel + sel_list.size();
-    // ---- phase A
-    int ti = e->tbegin(EIMP_KCLASS_DISTANCE);
-    HIPCHK(launch_distance(e->pd, e->d_tg.as<TgtDev>(), d_sel, (uint32_t)sel_list.size(), e->stream));
-    e->tend(ti);
-    ti = e->tbegin(EIMP_KCLASS_SELECT);
-    HIPCHK(launch_select(e->pd, e->d_tg.as<TgtDev>(), nt, e->stream));
-    e->tend(ti);
-    ti = e->tbegin(EIMP_KCLASS_MONOS);
-    HIPCHK(launch_monos(e->pd, e->d_tg.as<TgtDev>(), d_mono_list, (uint32_t)mono_list.size(), max_ww,
-                        e->stream));
-    e->tend(ti);
-    ti = e->tbegin(EIMP_KCLASS_SPLITS);
-    HIPCHK(launch_splits(e->pd, e->d_tg.as<TgtDev>(), nt, e->stream));
-    e->tend(ti);
-    e->h_stats.resize(4 * (size_t)nt);
-    HIPCHK(hipMemcpyAsync(e->h_stats.data(), e->d_stats.p, (size_t)nt * 16, hipMemcpyDeviceToHost,
-                          e->stream));
-    CHK(e->sync());
-    for (uint32_t t = 0; t < nt; t++) e->h_tg[t].S = e->h_stats[4 * t];
-    // ---- phase B
-    const bool digest = (flags & EIMP_BATCH_DIGEST) != 0;
-    CHK(run_build(e, nt, digest, batch_layout(flags)));
-    std::vector<uint64_t> dig(digest ? 2 * (size_t)nt : 0);
-    if (digest)
-        HIPCHK(hipMemcpyAsync(dig.data(), e->d_digest.p, (size_t)nt * 16, hipMemcpyDeviceToHost,
-                              e->stream));
-    CHK(e->sync());
-    for (uint32_t t = 0; t < nt; t++) {
-        const TgtDev &T = e->h_tg[t];
-        results[t].S = T.S;
-        results[t].Mp = T.Mp;
-        results[t].ncalls = e->h_stats[4 * t + 1];
-        results[t].K = T.K;
-        results[t].digest_pbwt = digest ? dig[2 * t] : 0;
-        results[t].digest_count0 = digest ? dig[2 * t + 1] : 0;
-    }
-    e->last_n = nt;
-    return EIMP_OK;
+    PoolMem mem(e);
+    return run_batch_impl(e, n, targets, cMmaxSplit, flags, results, mem, false);
 }
 
 int eimp_engine_run_cref_pbwt(eimp_engine *e, size_t n, const uint64_t *const *tgtData,
@@ -1005,7 +1167,8 @@ int eimp_engine_run_cref_pbwt(eimp_engine *e, size_t n, const uint64_t *const *t
                                               ": crefTOffsets disagree with the split bitmap");
         e->h_tg[t].S = S;
     }
-    CHK(run_build(e, nt, false, LAYOUT_GPU));
+    PoolMem mem(e);
+    CHK(run_build(e, nt, false, LAYOUT_GPU, mem));
     // results to the caller's host buffers, as src/GPUEngine.cu:182-211
     for (uint32_t t = 0; t < nt; t++) {
         const TgtDev &T = e->h_tg[t];
@@ -1025,7 +1188,7 @@ int eimp_engine_copy_target(eimp_engine *e, size_t t, uint32_t *besthaps, uint64
     if (!e || !e->initialized) return fail(EIMP_ERR_STATE, "engine not initialized");
     if (t >= e->last_n || t >= e->h_tg.size()) return fail(EIMP_ERR_INVALID, "target index out of range");
     CHK(set_device(e));
-    return copy_target_out(e->h_tg[t], e->d_tg.as<TgtDev>(), (uint32_t)t, e->Nad, e->stream, e->d_tmp,
+    return copy_target_out(e->h_tg[t], e->last_tg, (uint32_t)t, e->Nad, e->stream, e->d_tmp,
                            besthaps, bhflags, splitsites, splits, cMdiffs, count0, pbwt, cref_rows);
 }
 
@@ -1046,19 +1209,41 @@ int eimp_engine_ref_pbwt(eimp_engine *e, size_t m0, size_t nsites, uint32_t *row
         return fail(EIMP_ERR_INVALID, "site range outside the panel");
     if (nsites >= (1u << 31)) return fail(EIMP_ERR_UNSUPPORTED, "too many sites");
     const uint64_t N = e->Nad, Kpad = round_up(N, 64);
-    if (Kpad > panel_pbwt_max_k())
-        return fail(EIMP_ERR_UNSUPPORTED, "panel wider than the PBWT workgroup (" +
-                                              std::to_string(panel_pbwt_max_k()) + " haplotypes)");
+    // panels up to one workgroup's LDS run the LDS-resident kernel; wider ones
+    // the large-K kernel (order in global memory)
+    const bool big = Kpad > panel_pbwt_max_k();
+    if (big && (N > pbwt_big_max_k() || pbwt_big_lds_bytes((uint32_t)N) > 160 * 1024))
+        return fail(EIMP_ERR_UNSUPPORTED, "panel wider than " + std::to_string(pbwt_big_max_k()) +
+                                              " haplotypes");
     CHK(set_device(e));
     const uint64_t nblk = (nsites + 15) / 16, RS = 2 * ((N + 31) / 32);
-    CHK(e->p_blocks.ensure(nblk * Kpad * 2, "panel PBWT blocks"));
+    if (!big) CHK(e->p_blocks.ensure(nblk * Kpad * 2, "panel PBWT blocks"));
     CHK(e->p_rows.ensure(nsites * RS * 4, "panel PBWT rows"));
     CHK(e->p_c0.ensure(nsites * 4, "panel PBWT count0"));
     if (absperm) CHK(e->p_abs.ensure((nsites + 1) * N * 4, "panel PBWT permutations"));
+    if (big) {
+        CHK(e->d_ord.ensure(2 * Kpad * 4, "panel PBWT orders"));
+        CHK(e->d_big.ensure(sizeof(BigItem), "panel PBWT item"));
+        BigItem B{};
+        B.rows = e->site.as<uint64_t>() + m0 * (e->refTCap / 8);
+        B.rstride = e->refTCap / 8;
+        B.out = e->p_rows.as<uint32_t>();
+        B.count0 = e->p_c0.as<int32_t>();
+        B.absperm = absperm ? e->p_abs.as<int32_t>() : nullptr;
+        B.ord0 = e->d_ord.as<uint32_t>();
+        B.ord1 = B.ord0 + Kpad;
+        B.K = (uint32_t)N;
+        B.Mp = (uint32_t)nsites;
+        B.RS = (uint32_t)RS;
+        HIPCHK(hipMemcpyAsync(e->d_big.p, &B, sizeof(BigItem), hipMemcpyHostToDevice, e->stream));
+    }
     hipEvent_t a = e->get_event(), b = e->get_event();
     if (a) (void)hipEventRecord(a, e->stream);
-    HIPCHK(launch_panel_pbwt(e->pd, m0, (uint32_t)nsites, e->p_blocks.as<uint16_t>(), e->p_rows.as<uint32_t>(),
-                             e->p_c0.as<int32_t>(), absperm ? e->p_abs.as<int32_t>() : nullptr, e->stream));
+    if (big)
+        HIPCHK(launch_pbwt_big(e->d_big.as<BigItem>(), 1, (uint32_t)N, e->stream));
+    else
+        HIPCHK(launch_panel_pbwt(e->pd, m0, (uint32_t)nsites, e->p_blocks.as<uint16_t>(), e->p_rows.as<uint32_t>(),
+                                 e->p_c0.as<int32_t>(), absperm ? e->p_abs.as<int32_t>() : nullptr, e->stream));
     if (b) (void)hipEventRecord(b, e->stream);
     HIPCHK(hipStreamSynchronize(e->stream));
     if (device_ms) {

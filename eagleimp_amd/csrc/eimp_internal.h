@@ -103,6 +103,20 @@ struct PbwtItem {
     uint32_t t, dir;
 };
 
+// One item of the large-K PBWT kernel (order in global memory): one
+// direction of one target, or the panel PBWT.
+struct BigItem {
+    const uint64_t *rows;    // row r of this item's rows (bit k = haplotype k) at rows + r * rstride
+    uint64_t rstride;        // uint64 words per row
+    uint32_t *out;           // 2 x Mp (or Mp) rows of RS uint32, this direction's first row
+    int32_t *count0;         // Mp zero counts (forward items), else null
+    int32_t *absperm;        // (Mp + 1) x K absolute orders (panel PBWT) or null
+    uint32_t *ord0, *ord1;   // Kpad indexes each
+    uint64_t *digest;        // [0] += digest of the rows, or null
+    uint64_t gbase;          // global index of this direction's first logical word (digest)
+    uint32_t K, Mp, RS, rev, raw, pad;
+};
+
 // Output pool and queues of a stream context, for k_plan.  Each target gets
 // one contiguous region of the pool: cref blocks, count0, PBWT rows.
 struct PlanArgs {

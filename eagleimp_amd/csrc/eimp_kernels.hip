@@ -1200,6 +1200,149 @@ void k_pbwt(const TgtDev *__restrict__ tg, const PbwtItem *__restrict__ items) {
 }
 
 // ---------------------------------------------------------------------------
+// Compressed PBWT for K beyond the LDS-resident kernels (src/PBWT.cpp:207-285,
+// the same rows): the order lives in global memory (two arrays of Kpad
+// haplotype indexes, swapped every step; L2-resident at these sizes) and only
+// the current row's K bits are staged in LDS.  One workgroup of BIG_NT
+// threads per item; wave w takes chunks w, w + NW, ... of 64 positions.  A
+// step: stage row r; pass 1 counts the zeros of every chunk (ballots) into
+// LDS; a block scan turns them into zeros-before-chunk; pass 2 recomputes
+// each chunk's ballot, scatters every index to its place under the stable
+// partition (zeros first) and writes the chunk's (data, cumulative zeros)
+// pairs.  Positions K..Kpad-1 hold index K and read as ones (1-padding).
+// ---------------------------------------------------------------------------
+constexpr int BIG_NT = 1024;
+constexpr uint32_t kBigMaxK = 1u << 19;  // LDS: K/8 bytes of row + K/16 of chunk counts
+
+
+__device__ __forceinline__ uint32_t big_bit(const uint32_t *rowb, uint32_t idx, uint32_t K) {
+    return idx < K ? (rowb[idx >> 5] >> (idx & 31)) & 1u : 1u;
+}
+
+__global__ __launch_bounds__(BIG_NT) void k_pbwt_big(const BigItem *__restrict__ items) {
+    extern __shared__ uint32_t smem[];
+    constexpr uint32_t NW = BIG_NT / 64;
+    const BigItem I = items[blockIdx.x];
+    const uint32_t K = I.K, Mp = I.Mp, RS = I.RS;
+    const uint32_t Kpad = (K + 63) / 64 * 64, nch = Kpad / 64;
+    const uint32_t nw32 = (K + 31) / 32, npairs = nw32, KW = 2 * npairs;
+    uint32_t *rowb = smem;               // nw32 words: the current row
+    uint32_t *cz = smem + (nw32 + 1) / 2 * 2;  // nch: zeros per chunk, then zeros before it
+    __shared__ uint32_t s_part[NW + 1];
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, ln = lane_id();
+    gptr_t<uint32_t> ordA = gmem(I.ord0), ordB = gmem(I.ord1);
+    for (uint32_t q = tid; q < Kpad; q += BIG_NT) {
+        ordA[q] = q < K ? q : K;
+        if (I.absperm && q < K) gmem(I.absperm)[q] = (int32_t)q;
+    }
+    {
+        // the padding words of every row, stored once (as pbwt_steps)
+        const uint32_t pw0 = I.raw ? 2 * nch : 2 * npairs;
+        const uint32_t fw = I.raw ? ~0u : 0u;
+        for (uint32_t rr = tid; rr < Mp; rr += BIG_NT) {
+            gptr_t<uint32_t> prow = gmem(I.out) + (uint64_t)rr * RS;
+            for (uint32_t w = pw0; w < RS; w++) prow[w] = fw;
+        }
+    }
+    uint64_t dsum = 0;
+    __syncthreads();
+    for (uint32_t step = 0; step < Mp; step++) {
+        const uint32_t r = I.rev ? Mp - 1 - step : step;
+        const gptr_t<const uint32_t> src = (gptr_t<const uint32_t>)(I.rows + (uint64_t)r * I.rstride);
+        for (uint32_t q = tid; q < nw32; q += BIG_NT) rowb[q] = src[q];
+        __syncthreads();
+        // pass 1: zeros of every chunk
+        for (uint32_t ch = wave; ch < nch; ch += NW) {
+            const uint32_t idx = ordA[ch * 64 + ln];
+            const uint64_t m = __ballot(big_bit(rowb, idx, K));
+            if (ln == 0) cz[ch] = 64u - (uint32_t)__popcll(m);
+        }
+        __syncthreads();
+        // exclusive scan of cz over the chunks: thread tid owns a contiguous run
+        const uint32_t per = (nch + BIG_NT - 1) / BIG_NT;
+        const uint32_t c0 = min(nch, tid * per), c1 = min(nch, c0 + per);
+        uint32_t mine = 0;
+        for (uint32_t c = c0; c < c1; c++) mine += cz[c];
+        const uint32_t incl = wave_scan_incl(mine);
+        if (ln == 63) s_part[wave] = incl;
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (uint32_t w = 0; w < NW; w++) {
+            const uint32_t v = s_part[w];
+            total += v;
+            before += w < wave ? v : 0u;
+        }
+        uint32_t run = before + incl - mine;
+        for (uint32_t c = c0; c < c1; c++) {
+            const uint32_t z = cz[c];
+            cz[c] = run;
+            run += z;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            if (I.count0) gmem(I.count0)[r] = (int32_t)total;
+        }
+        // pass 2: scatter the order, write the row's pairs
+        const gptr_t<uint32_t> row = gmem(I.out) + (uint64_t)step * RS;
+        for (uint32_t ch = wave; ch < nch; ch += NW) {
+            const uint32_t pos = ch * 64 + ln;
+            const uint32_t idx = ordA[pos];
+            const uint32_t b = big_bit(rowb, idx, K);
+            const uint64_t m = __ballot(b);
+            const uint32_t zb = __builtin_amdgcn_readfirstlane(cz[ch]);
+            const uint32_t t = mbcnt(m);  // ones below me in the chunk
+            const uint32_t dest = b ? total + (ch * 64u - zb) + t : zb + (ln - t);
+            ordB[dest] = idx;
+            if (I.absperm && dest < K) gmem(I.absperm)[(uint64_t)(step + 1) * K + dest] = (int32_t)idx;
+            const uint32_t lo = (uint32_t)m, hi = (uint32_t)(m >> 32);
+            const uint32_t o1 = zb + 32u - (uint32_t)__popc(lo);
+            const uint32_t o3 = zb + 64u - (uint32_t)__popcll(m);
+            const bool second = 2 * ch + 1 < npairs;
+            if (ln == 0) {
+                if (I.raw) {
+                    row[2 * ch] = lo;
+                    row[2 * ch + 1] = hi;
+                } else {
+                    row[4 * ch] = lo;
+                    row[4 * ch + 1] = o1;
+                    if (second) {
+                        row[4 * ch + 2] = hi;
+                        row[4 * ch + 3] = o3;
+                    }
+                }
+                if (I.digest) {
+                    const uint64_t g = I.gbase + (uint64_t)step * KW + 4ull * ch;
+                    dsum += (uint64_t)dig_word(g, lo) + dig_word(g + 1, o1);
+                    if (second) dsum += (uint64_t)dig_word(g + 2, hi) + dig_word(g + 3, o3);
+                }
+            }
+        }
+        __syncthreads();  // the new order is complete (and cz free) before the next step
+        const gptr_t<uint32_t> tmp = ordA;
+        ordA = ordB;
+        ordB = tmp;
+    }
+    if (I.digest && ln == 0) atomicAdd((unsigned long long *)I.digest, (unsigned long long)dsum);
+}
+
+static hipError_t raise_lds_limit(const void *fn, size_t lds);
+
+size_t pbwt_big_lds_bytes(uint32_t K) {
+    const uint32_t nw32 = (K + 31) / 32, nch = (K + 63) / 64;
+    return (size_t)((nw32 + 1) / 2 * 2 + nch) * 4;
+}
+uint32_t pbwt_big_max_k() { return kBigMaxK; }
+
+hipError_t launch_pbwt_big(const BigItem *items, uint32_t nitems, uint32_t maxK, hipStream_t s) {
+    if (!nitems) return hipSuccess;
+    const size_t lds = pbwt_big_lds_bytes(maxK);
+    hipError_t e = raise_lds_limit((const void *)k_pbwt_big, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_pbwt_big, dim3(nitems), dim3(BIG_NT), lds, s, items);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Panel PBWT for imputation (PBWT(refT, true), src/PBWT.cpp:53-71,131-153):
 // one workgroup, forward only, K = all panel haplotypes, the absolute order
 // after every site and count0 per site written out.
