@@ -70,6 +70,16 @@ hipError_t launch_pbwt_persist(int v, const TgtDev *tg, BatchDev *B, const PbwtI
 
 using namespace eimp;
 
+// Scratch of eimp_engine_run_batch_noalloc: every device buffer of a batch
+// except its inputs (genotype words) and outputs (rows, count0).  Each device
+// that runs the library holds its own copy until the process ends.  In host
+// code the name is only a stand-in: the device address of the current
+// device's copy comes from hipGetSymbolAddress (scratch_for_device) and only
+// that address is passed to kernels and copies.
+static constexpr size_t kScratchBytes = 64ull << 20;
+static constexpr size_t kNoAllocMaxTargets = 256;
+static __device__ __align__(256) uint64_t scratch[kScratchBytes / 8];
+
 namespace {
 
 thread_local std::string g_err;
@@ -182,6 +192,7 @@ struct eimp_engine {
     uint32_t last_n = 0;
     TgtDev *last_tg = nullptr;    // device target table of the last batch
     uint64_t *last_dig = nullptr; // its digests
+    uint64_t *scratch = nullptr;  // this device's copy of the no-allocation scratch array
     // timing
     bool timing = false;
     double ms[EIMP_NKCLASS] = {};
@@ -308,8 +319,10 @@ int ensure_pmono(eimp_engine *e) {
     return e->sync();
 }
 
-// Where one batch's device buffers come from (the engine's grow-only pools for
-// eimp_engine_run_batch and the drop-in call).
+// Where one batch's device buffers come from: the engine's grow-only pools
+// (eimp_engine_run_batch, the drop-in call) or a bump allocator over the
+// file-scope scratch array with caller-provided outputs
+// (eimp_engine_run_batch_noalloc).
 enum MemSlot : int {
     MS_TG, MS_NERR, MS_BH, MS_BHF, MS_MONO, MS_SS, MS_SPL, MS_CMD, MS_STATS, MS_TLIST,
     MS_CREF, MS_DIG, MS_CITEMS, MS_PITEMS, MS_CROWS, MS_ORD, MS_BIG
@@ -372,6 +385,60 @@ struct PoolMem final : BatchMem {
         return EIMP_OK;
     }
 };
+
+// Bump allocation over one device's copy of the scratch array; the outputs
+// are the caller's (rows[t]: 2*Mp*RS uint32, count0[t]: Mp int32, Mp <= max_Mp).
+struct ScratchMem final : BatchMem {
+    char *base;
+    size_t used = 0;
+    uint32_t *const *rows;
+    int32_t *const *c0;
+    size_t max_Mp;
+    ScratchMem(uint64_t *scr, uint32_t *const *rows_, int32_t *const *c0_, size_t max_Mp_)
+        : base(reinterpret_cast<char *>(scr)), rows(rows_), c0(c0_), max_Mp(max_Mp_) {}
+    int get(MemSlot, size_t bytes, const char *what, void **p) override {
+        const size_t b = round_up(std::max<size_t>(bytes, 1), 256);
+        if (used + b > kScratchBytes)
+            return fail(EIMP_ERR_UNSUPPORTED, std::string("batch needs more than the ") +
+                                                  std::to_string(kScratchBytes >> 20) +
+                                                  " MiB no-allocation scratch (at " + what + ")");
+        *p = base + used;
+        used += b;
+        return EIMP_OK;
+    }
+    int place_outputs(std::vector<TgtDev> &tg, uint32_t n) override {
+        for (uint32_t t = 0; t < n; t++) {
+            if (tg[t].Mp > max_Mp)
+                return fail(EIMP_ERR_UNSUPPORTED, "target " + std::to_string(t) + ": Mp = " +
+                                                      std::to_string(tg[t].Mp) + " exceeds max_Mp = " +
+                                                      std::to_string(max_Mp));
+            if (!rows[t] || !c0[t])
+                return fail(EIMP_ERR_INVALID, "target " + std::to_string(t) + ": null output buffer");
+            tg[t].out = rows[t];
+            tg[t].count0 = c0[t];
+        }
+        return EIMP_OK;
+    }
+    bool may_alloc() const override { return false; }
+};
+
+// Device address of the current device's scratch copy, looked up once per
+// device (the device must be current).
+int scratch_for_device(int dev, uint64_t **out) {
+    static std::mutex mu;
+    static std::map<int, uint64_t *> addr;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = addr.find(dev);
+    if (it == addr.end()) {
+        void *ptr = nullptr;
+        const hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (err != hipSuccess || !ptr)
+            return fail(EIMP_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(err));
+        it = addr.emplace(dev, static_cast<uint64_t *>(ptr)).first;
+    }
+    *out = it->second;
+    return EIMP_OK;
+}
 
 // Phase B for targets whose phase-A fields (bh, splitsites, S) are in place.
 // Sizes outputs, builds cref + PBWT in the given output layout (LAYOUT_*).
@@ -981,6 +1048,14 @@ int eimp_engine_create(int device_index, eimp_engine **engine) {
         delete e;
         return fail(EIMP_ERR_HIP, "stream creation failed");
     }
+    // this device's copy of the no-allocation scratch (device current)
+    const int rc = scratch_for_device(device_index, &e->scratch);
+    if (rc != EIMP_OK) {
+        const std::string msg = g_err;
+        (void)hipStreamDestroy(e->stream);
+        delete e;
+        return fail(rc, msg);
+    }
     *engine = e;
     return EIMP_OK;
 }
@@ -1088,7 +1163,26 @@ int eimp_engine_set_panel_extras(eimp_engine *e, const float *cM, const uint64_t
     return EIMP_OK;
 }
 
-
+int eimp_engine_run_batch_noalloc(eimp_engine *e, size_t n, const eimp_target *targets,
+                                  float cMmaxSplit, uint32_t flags, size_t max_Mp,
+                                  uint32_t *const *rows, int32_t *const *count0,
+                                  eimp_target_result *results) {
+    if (!e || !e->initialized) return fail(EIMP_ERR_STATE, "engine not initialized");
+    if (n == 0) return EIMP_OK;
+    if (n > kNoAllocMaxTargets)
+        return fail(EIMP_ERR_UNSUPPORTED, "at most " + std::to_string(kNoAllocMaxTargets) +
+                                              " targets per no-allocation batch");
+    if (!rows || !count0) return fail(EIMP_ERR_INVALID, "null output arrays");
+    if (!e->scratch) return fail(EIMP_ERR_STATE, "no scratch address for this device");
+    CHK(set_device(e));
+    // event timing would create events: off for this call
+    const bool timing = e->timing;
+    e->timing = false;
+    ScratchMem mem(e->scratch, rows, count0, max_Mp);
+    const int rc = run_batch_impl(e, n, targets, cMmaxSplit, flags, results, mem, true);
+    e->timing = timing;
+    return rc;
+}
 
 int eimp_engine_run_batch(eimp_engine *e, size_t n, const eimp_target *targets, float cMmaxSplit,
                           uint32_t flags, eimp_target_result *results) {

@@ -36,7 +36,7 @@ EXPORTS = [
     "eimp_engine_ref_pbwt", "eimp_phase_default_params", "eimp_phase_target", "eimp_phase_batch",
     "eimp_qref_open", "eimp_qref_close", "eimp_qref_metadata", "eimp_qref_variant",
     "eimp_qref_read_sites", "eimp_engine_copy_monos", "eimp_stream_copy_monos",
-    "eimp_stream_digest_outputs",
+    "eimp_stream_digest_outputs", "eimp_engine_run_batch_noalloc",
 ]
 
 _P = C.c_void_p
@@ -93,6 +93,8 @@ def lib():
         L.eimp_engine_copy_monos.argtypes = [_P, _SZ, _P]
         L.eimp_stream_copy_monos.argtypes = [_P, _SZ, _P]
         L.eimp_stream_digest_outputs.argtypes = [_P, _P]
+        L.eimp_engine_run_batch_noalloc.argtypes = [_P, _SZ, _P, C.c_float, C.c_uint32, _SZ, _P,
+                                                    _P, _P]
         _LIB = L
     return _LIB
 
@@ -228,6 +230,32 @@ class Engine:
         flags = ((EIMP_BATCH_DIGEST if digest else 0) | (EIMP_BATCH_COMPACT if compact else 0)
                  | (EIMP_BATCH_RAW if raw else 0))
         _check(lib().eimp_engine_run_batch(self._h, n, ts, C.c_float(cMmaxSplit), flags, rs))
+        arr = np.frombuffer(rs, dtype=np.uint64).reshape(n, 6)
+        self._last = arr.copy()
+        self._last_layout = layout_name(compact, raw)
+        self._last_W = [int(w) for w in Ws]
+        return BatchResult(S=arr[:, 0].copy(), Mp=arr[:, 1].copy(), ncalls=arr[:, 2].copy(),
+                           K=arr[:, 3].copy(), dig_pbwt=arr[:, 4].copy(),
+                           dig_count0=arr[:, 5].copy())
+
+    def run_batch_noalloc(self, gt_dev, s0s, Ws, Ks, max_Mp, rows_dev, count0_dev,
+                          ignore=None, cMmaxSplit=0.125, digest=True, compact=False,
+                          raw=False) -> BatchResult:
+        """eimp_engine_run_batch_noalloc: device addresses (ints) of each
+        target's genotype words, output rows (2*max_Mp*rowstride uint32) and
+        count0 (max_Mp int32); nothing is allocated by the call."""
+        n = len(gt_dev)
+        ts = (EimpTarget * n)()
+        for i in range(n):
+            ts[i] = EimpTarget(int(gt_dev[i]), int(s0s[i]), int(Ws[i]), int(Ks[i]),
+                               -1 if ignore is None else int(ignore[i]))
+        rs = (EimpResult * n)()
+        A = C.c_void_p * n
+        flags = ((EIMP_BATCH_DIGEST if digest else 0) | (EIMP_BATCH_COMPACT if compact else 0)
+                 | (EIMP_BATCH_RAW if raw else 0))
+        _check(lib().eimp_engine_run_batch_noalloc(
+            self._h, n, ts, C.c_float(cMmaxSplit), flags, int(max_Mp),
+            A(*[int(p) for p in rows_dev]), A(*[int(p) for p in count0_dev]), rs))
         arr = np.frombuffer(rs, dtype=np.uint64).reshape(n, 6)
         self._last = arr.copy()
         self._last_layout = layout_name(compact, raw)

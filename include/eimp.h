@@ -143,6 +143,30 @@ typedef struct eimp_target_result {
 int eimp_engine_run_batch(eimp_engine *engine, size_t n, const eimp_target *targets,
                           float cMmaxSplit, uint32_t flags, eimp_target_result *results);
 
+/* eimp_engine_run_batch without any device (or pinned host) allocation.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  The caller passes inputs and outputs in device memory:
+ * targets[t].gt points to the target's GenotypeVector words ON THE DEVICE,
+ * rows[t] to 2*Mp*rowstride uint32 and count0[t] to Mp int32 of device
+ * memory (rowstride as the layout flag chooses; Mp <= max_Mp, which the
+ * caller sizes the buffers by).  Everything else (distances, best haps,
+ * split data, condensed reference, tables) lives in a fixed 64 MiB device
+ * scratch array that every device running the library holds until the
+ * process ends; a batch needing more, more than 256 targets, or an Mp above
+ * max_Mp fails with EIMP_ERR_UNSUPPORTED.  The panel parts a batch reads must
+ * be resident already: the mono bits are built by
+ * eimp_engine_set_panel_extras, the hap-major rows (targets with K < Ncorr or
+ * windows) by passing hapData there or by one earlier eimp_engine_run_batch;
+ * otherwise EIMP_ERR_STATE.  All calls on one device share that device's
+ * scratch: calls that overlap on one device (two engines, two threads) race.
+ * Results, digests and eimp_engine_copy_target behave as after
+ * eimp_engine_run_batch (until the next call on the device). */
+int eimp_engine_run_batch_noalloc(eimp_engine *engine, size_t n, const eimp_target *targets,
+                                  float cMmaxSplit, uint32_t flags, size_t max_Mp,
+                                  uint32_t *const *rows, int32_t *const *count0,
+                                  eimp_target_result *results);
+
 /* Copies outputs of target t of the last batch (they stay on the device
  * until the next batch or drop-in call).  Any pointer may be NULL.  Sizes: besthaps K uint32, bhflags
  * roundToMultiple(NrefhapsAD,512)/64 uint64, splitsites S uint32, splits
