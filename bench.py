@@ -712,7 +712,12 @@ def batch_bench(args) -> dict:
             "kernel_bytes": "rows written (8*ceil(K/32) per row and direction) + condensed "
                             "reference read (K/8 per row and direction)",
             "traffic_source": traffic_src,
-            "limiter": "VALU issue (SQ counters)" if sq else None,
+            "traffic_over_alg": None if traffic is None else round(traffic / balg_launch, 3),
+            "traffic_over_kernel_bytes": None if traffic is None else round(traffic / kbytes_launch, 3),
+            "limiter": None if sq is None else (
+                f"step latency (VALU busy {100 * sq['valu_busy']:.1f}%, "
+                f"{sq['valu_per_wave_step']} VALU / {sq['salu_per_wave_step']} SALU / "
+                f"{sq['lds_per_wave_step']} LDS instructions per wave-step)"),
             "sq": None if sq is None else {k: sq[k] for k in sq if k not in ("kernel", "workload")},
         },
         "kernel_ms": {k: round(v[0], 3) for k, v in tim.items()},
