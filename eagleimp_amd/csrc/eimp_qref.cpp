@@ -17,6 +17,12 @@
 //
 // The file is memory-mapped; variant offsets are indexed once at open; site
 // rows decode in parallel straight into the engine's site-major layout.
+//
+// Sample masking (participant withdrawals, src/VCFData.cpp:2139-2208 and
+// 2418-2530; sample lists src/VCFData.cpp:3698-3746): the masked samples'
+// two haplotype bits leave every row, the later haplotypes move down, the
+// haploid flags of the kept samples stay, and each variant's allele
+// frequency is corrected by the variant alleles the masked samples carried.
 #include <fcntl.h>
 #include <sys/mman.h>
 #include <sys/stat.h>
@@ -24,7 +30,10 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cctype>
+#include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -47,6 +56,11 @@ struct eimp_qref {
     std::vector<const char *> alleles;  // 2 per variant, then
     std::vector<const char *> ids;      // 1 per variant (zero-terminated, inside the mapping)
     std::vector<uint64_t> var_off;      // offset of each variant's encoded size
+    // as stored in the file, before any mask
+    uint64_t file_Nref = 0, file_NrefHap = 0;
+    // masked samples (ascending) and whether each is haploid
+    std::vector<uint64_t> mask;
+    std::vector<uint8_t> mask_haploid;
     ~eimp_qref() {
         if (base) munmap(const_cast<uint8_t *>(base), size);
         if (fd >= 0) close(fd);
@@ -102,6 +116,92 @@ bool rle_decode(const uint8_t *enc, size_t n, uint64_t *out, size_t words) {
     return pos >= total || pos % 64 == 0;
 }
 
+bool file_haploid(const eimp_qref *q, uint64_t s) {
+    if (q->haploid) return q->haploid[s] != 0;
+    return q->file_NrefHap == q->file_Nref;  // all haploid or all diploid (src/VCFData.cpp:2196-2208)
+}
+
+// Bit `pos` onwards of a vector of `words` words, 64 bits (zero past its end).
+inline uint64_t bits_at(const uint64_t *v, size_t words, uint64_t pos) {
+    const uint64_t w = pos >> 6, sh = pos & 63;
+    if (w >= words) return 0;
+    uint64_t x = v[w] >> sh;
+    if (sh && w + 1 < words) x |= v[w + 1] << (64 - sh);
+    return x;
+}
+
+// OR bits [s, s+len) of src into dst starting at bit d (dst pre-zeroed, dst
+// bits past dst_words*64 dropped).
+void copy_bits(const uint64_t *src, size_t src_words, uint64_t s, uint64_t *dst, size_t dst_words,
+               uint64_t d, uint64_t len) {
+    const uint64_t dend = std::min<uint64_t>(d + len, (uint64_t)dst_words * 64);
+    while (d < dend) {
+        const uint64_t w = d >> 6, sh = d & 63;
+        const uint64_t take = std::min<uint64_t>(64 - sh, dend - d);
+        uint64_t x = bits_at(src, src_words, s);
+        if (take < 64) x &= (1ull << take) - 1;
+        dst[w] |= x << sh;
+        d += take;
+        s += take;
+    }
+}
+
+// One variant's record (size word at p) decoded into full (q->hap_words
+// words, all 2*file_Nref haplotypes).  Returns false for a truncated code.
+bool decode_record(const eimp_qref *q, const uint8_t *p, uint64_t *full, bool *raw) {
+    const uint64_t enc = u64_at(p);
+    *raw = enc == 0;
+    if (enc == 0) {
+        std::memcpy(full, p + 8, q->hap_words * 8);
+        return true;
+    }
+    std::fill(full, full + q->hap_words, 0ull);
+    return rle_decode(p + 8, enc, full, q->hap_words);
+}
+
+// Variant alleles of the masked samples in one decoded record.  Run-length
+// coded records count a haploid sample's allele once, from its second bit
+// after a set first bit (src/utils.h:245-262: only (1,1) counts); raw
+// records count one for any set bit and a second for a diploid sample with
+// both set (src/VCFData.cpp:2468-2475: a haploid (1,0) or (0,1) counts one).
+uint64_t masked_alleles(const eimp_qref *q, const uint64_t *full, bool raw) {
+    uint64_t c = 0;
+    for (size_t i = 0; i < q->mask.size(); i++) {
+        const uint64_t h = 2 * q->mask[i];
+        const uint64_t a = (full[h >> 6] >> (h & 63)) & 1u, b = (full[h >> 6] >> ((h & 63) + 1)) & 1u;
+        if (q->mask_haploid[i])
+            c += raw ? (a | b) : (a & b);
+        else
+            c += a + b;
+    }
+    return c;
+}
+
+// The kept haplotypes of a decoded record, in order, into dst (pre-zeroed).
+void compact_masked(const eimp_qref *q, const uint64_t *full, uint64_t *dst, size_t dst_words) {
+    uint64_t s = 0, d = 0;
+    for (uint64_t ms : q->mask) {
+        const uint64_t h = 2 * ms;
+        copy_bits(full, q->hap_words, s, dst, dst_words, d, h - s);
+        d += h - s;
+        s = h + 2;
+    }
+    copy_bits(full, q->hap_words, s, dst, dst_words, d, 2 * q->file_Nref - s);
+}
+
+// The allele frequency the reference holds after masking
+// (src/VCFData.cpp:2426-2432, 2524-2529): new_ac = old_an * old_af -
+// masked alleles, af = new_ac / new_an, in single precision.  The reference
+// is built with -march=native at -O3 (src/CMakeLists.txt:165, README), where
+// GCC contracts the product and the subtraction into one fused
+// multiply-subtract on every host with FMA; fmaf gives that single rounding.
+float corrected_af(const eimp_qref *q, float old_af, uint64_t masked) {
+    const uint64_t old_an = 2 * q->file_Nref - q->file_NrefHap;
+    const uint64_t new_an = 2 * q->info.Nref - q->info.NrefHap;
+    const float new_ac = std::fma((float)old_an, old_af, -(float)masked);
+    return new_ac / (float)new_an;
+}
+
 }  // namespace
 
 extern "C" {
@@ -138,6 +238,8 @@ int eimp_qref_open(const char *path, eimp_qref **out, eimp_qref_info *info) {
     q->info.Mref = u64_at(b + 24);
     q->info.MrefMA = u64_at(b + 32);
     const uint64_t N = q->info.Nref, M = q->info.Mref;
+    q->file_Nref = N;
+    q->file_NrefHap = q->info.NrefHap;
     if (q->info.NrefHap > N) return bad("more haploid samples than samples");
     if (N > (1ull << 40) || M > (1ull << 40)) return bad("implausible sizes");
     q->hap_words = (2 * N + 511) / 512 * 8;
@@ -211,12 +313,16 @@ int eimp_qref_metadata(eimp_qref *q, int64_t *positions, float *af, uint8_t *hap
     const uint64_t N = q->info.Nref, M = q->info.Mref;
     if (positions && M) std::memcpy(positions, q->base + q->off_pos, 8 * M);
     if (af && M) std::memcpy(af, q->base + q->off_af, 4 * M);
-    if (haploid) {
-        if (q->haploid) {
-            for (uint64_t i = 0; i < N; i++) haploid[i] = q->haploid[i] ? 1 : 0;
-        } else {  // all haploid or all diploid (src/VCFData.cpp:2167-2168)
-            std::memset(haploid, (q->info.NrefHap == N && N > 0) ? 1 : 0, N);
+    if (haploid) {  // the kept samples' flags, in order (src/VCFData.cpp:2170-2192)
+        size_t k = 0, mi = 0;
+        for (uint64_t s = 0; s < q->file_Nref; s++) {
+            if (mi < q->mask.size() && q->mask[mi] == s) {
+                mi++;
+                continue;
+            }
+            haploid[k++] = file_haploid(q, s) ? 1 : 0;
         }
+        (void)N;
     }
     if (ma_flags && M) std::memcpy(ma_flags, q->base + q->off_ma, (M + 63) / 64 * 8);
     return EIMP_OK;
@@ -232,16 +338,21 @@ int eimp_qref_variant(eimp_qref *q, size_t m, const char **ref, const char **alt
     return EIMP_OK;
 }
 
-int eimp_qref_read_sites(eimp_qref *q, size_t m0, size_t n, uint64_t *rows, size_t row_words,
-                         int threads) {
+int eimp_qref_read_sites_af(eimp_qref *q, size_t m0, size_t n, uint64_t *rows, size_t row_words,
+                            float *af, uint64_t *masked, int threads) {
     if (!q) return fail(EIMP_ERR_INVALID, "qref: null handle");
     if (n == 0) return EIMP_OK;
-    if (!rows) return fail(EIMP_ERR_INVALID, "qref: null rows");
     if (m0 > q->info.Mref || n > q->info.Mref - m0)
         return fail(EIMP_ERR_INVALID, "qref: sites past the last variant");
     const size_t need_words = (2 * q->info.Nref + 63) / 64;
-    if (row_words < need_words)
+    if (rows && row_words < need_words)
         return fail(EIMP_ERR_INVALID, "qref: row_words below ceil(2*Nref/64)");
+    const bool masking = !q->mask.empty();
+    if (!masking && !rows) {  // nothing to decode
+        if (af) std::memcpy(af, q->base + q->off_af + 4 * m0, 4 * n);
+        if (masked) std::fill(masked, masked + n, 0ull);
+        return EIMP_OK;
+    }
     const size_t HW = q->hap_words;
     size_t nt = threads > 0 ? (size_t)threads : std::max(1u, std::thread::hardware_concurrency());
     nt = std::min(nt, n);
@@ -251,21 +362,30 @@ int eimp_qref_read_sites(eimp_qref *q, size_t m0, size_t n, uint64_t *rows, size
         std::vector<uint64_t> tmp(HW);
         for (size_t i; (i = next.fetch_add(1)) < n;) {
             const uint64_t m = m0 + i;
-            const uint8_t *p = q->base + q->var_off[m];
-            const uint64_t enc = u64_at(p);
-            uint64_t *dst = rows + i * row_words;
-            if (enc == 0) {
-                std::memcpy(tmp.data(), p + 8, HW * 8);
-            } else {
-                std::fill(tmp.begin(), tmp.end(), 0ull);
-                if (!rle_decode(p + 8, enc, tmp.data(), HW)) {
-                    int64_t expect = -1;
-                    bad.compare_exchange_strong(expect, (int64_t)m);
-                    continue;
-                }
+            bool raw;
+            if (!decode_record(q, q->base + q->var_off[m], tmp.data(), &raw)) {
+                int64_t expect = -1;
+                bad.compare_exchange_strong(expect, (int64_t)m);
+                continue;
             }
-            const size_t c = std::min(row_words, HW);
-            std::memcpy(dst, tmp.data(), c * 8);
+            float f;
+            std::memcpy(&f, q->base + q->off_af + 4 * m, 4);
+            uint64_t cnt = 0;
+            if (masking) {
+                cnt = masked_alleles(q, tmp.data(), raw);
+                f = corrected_af(q, f, cnt);
+            }
+            if (af) af[i] = f;
+            if (masked) masked[i] = cnt;
+            if (!rows) continue;
+            uint64_t *dst = rows + i * row_words;
+            if (masking) {
+                std::fill(dst, dst + row_words, 0ull);
+                compact_masked(q, tmp.data(), dst, std::min(row_words, need_words));
+            } else {
+                const size_t c = std::min(row_words, HW);
+                std::memcpy(dst, tmp.data(), c * 8);
+            }
             // bits past the 2*Nref haplotypes are padding: zero
             const uint64_t nb = 2 * q->info.Nref;
             if (nb % 64) dst[nb / 64] &= (1ull << (nb % 64)) - 1;
@@ -280,6 +400,99 @@ int eimp_qref_read_sites(eimp_qref *q, size_t m0, size_t n, uint64_t *rows, size
         return fail(EIMP_ERR_INVALID,
                     "qref: run-length code of variant " + std::to_string(bad.load()) + " is truncated");
     return EIMP_OK;
+}
+
+int eimp_qref_read_sites(eimp_qref *q, size_t m0, size_t n, uint64_t *rows, size_t row_words,
+                         int threads) {
+    if (!q) return fail(EIMP_ERR_INVALID, "qref: null handle");
+    if (n && !rows && row_words) return fail(EIMP_ERR_INVALID, "qref: null rows");
+    return eimp_qref_read_sites_af(q, m0, n, rows, row_words, nullptr, nullptr, threads);
+}
+
+int eimp_qref_set_mask(eimp_qref *q, const uint64_t *samples, size_t n, eimp_qref_info *info) {
+    if (!q) return fail(EIMP_ERR_INVALID, "qref: null handle");
+    if (n && !samples) return fail(EIMP_ERR_INVALID, "qref: null sample list");
+    for (size_t i = 0; i < n; i++) {
+        if (samples[i] >= q->file_Nref)
+            return fail(EIMP_ERR_INVALID, "qref: masked sample " + std::to_string(samples[i]) +
+                                              " past the panel's " + std::to_string(q->file_Nref) +
+                                              " samples");
+        if (i && samples[i] <= samples[i - 1])
+            return fail(EIMP_ERR_INVALID, "qref: masked samples not strictly ascending");
+    }
+    q->mask.assign(samples, samples + n);
+    q->mask_haploid.resize(n);
+    uint64_t hap = 0;
+    for (size_t i = 0; i < n; i++) hap += q->mask_haploid[i] = file_haploid(q, samples[i]) ? 1 : 0;
+    q->info.Nref = q->file_Nref - n;
+    q->info.NrefHap = q->file_NrefHap - hap;
+    q->info.hap_words = q->info.Mref ? (2 * q->info.Nref + 511) / 512 * 8 : 0;
+    if (info) *info = q->info;
+    return EIMP_OK;
+}
+
+int eimp_qref_parse_sample_list(const char *path, uint64_t *masked, size_t cap, size_t *nmasked,
+                                size_t *nsamples) {
+    if (!path || !nmasked || !nsamples) return fail(EIMP_ERR_INVALID, "sample list: null argument");
+    FILE *f = std::fopen(path, "r");
+    if (!f) return fail(EIMP_ERR_INVALID, std::string("sample list: cannot open ") + path);
+    const size_t L = std::strlen(path);
+    const bool bgen = L >= 7 && std::strcmp(path + L - 7, ".sample") == 0;
+    int64_t idx = bgen ? -2 : 0;  // a bgen .sample file has two header lines
+    size_t k = 0;
+    std::string line;
+    bool ok = true;
+    for (int c = 0; c != EOF;) {
+        line.clear();
+        while ((c = std::fgetc(f)) != EOF && c != '\n') line.push_back((char)c);
+        if (c == EOF && line.empty()) break;
+        if (idx >= 0) {
+            // first whitespace-separated token; the second one in a .sample file
+            auto token = [&](size_t &at) {
+                while (at < line.size() && std::isspace((unsigned char)line[at])) at++;
+                const size_t b = at;
+                while (at < line.size() && !std::isspace((unsigned char)line[at])) at++;
+                return line.substr(b, at - b);
+            };
+            size_t at = 0;
+            std::string sid = token(at);
+            if (!sid.empty() && bgen) {
+                sid = token(at);
+                if (sid.empty()) {
+                    ok = false;
+                    break;
+                }
+            }
+            // a withdrawal: 'W' or '-' followed by at least one digit and only digits
+            if (sid.size() >= 2 && (sid[0] == 'W' || sid[0] == '-') &&
+                std::all_of(sid.begin() + 1, sid.end(),
+                            [](char ch) { return std::isdigit((unsigned char)ch) != 0; })) {
+                if (masked && k < cap) masked[k] = (uint64_t)idx;
+                k++;
+            }
+        }
+        idx++;
+    }
+    std::fclose(f);
+    if (!ok) return fail(EIMP_ERR_INVALID, "sample list: not a valid bgen .sample file");
+    *nmasked = k;
+    *nsamples = idx > 0 ? (size_t)idx : 0;
+    if (masked && k > cap) return fail(EIMP_ERR_INVALID, "sample list: more withdrawals than cap");
+    return EIMP_OK;
+}
+
+int eimp_qref_mask_sample_list(eimp_qref *q, const char *path, eimp_qref_info *info) {
+    if (!q) return fail(EIMP_ERR_INVALID, "qref: null handle");
+    size_t nm = 0, ns = 0;
+    int rc = eimp_qref_parse_sample_list(path, nullptr, 0, &nm, &ns);
+    if (rc != EIMP_OK) return rc;
+    std::vector<uint64_t> ms(nm);
+    rc = eimp_qref_parse_sample_list(path, ms.data(), nm, &nm, &ns);
+    if (rc != EIMP_OK) return rc;
+    if (nm && ns != q->file_Nref)  // src/VCFData.cpp:2139-2142
+        return fail(EIMP_ERR_INVALID, "qref: sample list holds " + std::to_string(ns) +
+                                          " samples, the panel " + std::to_string(q->file_Nref));
+    return eimp_qref_set_mask(q, ms.data(), nm, info);
 }
 
 }  // extern "C"

@@ -35,7 +35,8 @@ EXPORTS = [
     "eimp_stream_destroy", "eimp_engine_mem_info", "eimp_stream_kernel_ms",
     "eimp_engine_ref_pbwt", "eimp_phase_default_params", "eimp_phase_target", "eimp_phase_batch",
     "eimp_qref_open", "eimp_qref_close", "eimp_qref_metadata", "eimp_qref_variant",
-    "eimp_qref_read_sites", "eimp_engine_copy_monos", "eimp_stream_copy_monos",
+    "eimp_qref_read_sites", "eimp_qref_set_mask", "eimp_qref_parse_sample_list",
+    "eimp_qref_mask_sample_list", "eimp_qref_read_sites_af", "eimp_engine_copy_monos", "eimp_stream_copy_monos",
     "eimp_stream_digest_outputs", "eimp_engine_run_batch_noalloc",
 ]
 

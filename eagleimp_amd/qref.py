@@ -33,6 +33,12 @@ def _bind():
         L.eimp_qref_variant.argtypes = [_P, C.c_size_t, C.POINTER(C.c_char_p),
                                         C.POINTER(C.c_char_p), C.POINTER(C.c_char_p)]
         L.eimp_qref_read_sites.argtypes = [_P, C.c_size_t, C.c_size_t, _P, C.c_size_t, C.c_int]
+        L.eimp_qref_set_mask.argtypes = [_P, _P, C.c_size_t, C.POINTER(QrefInfo)]
+        L.eimp_qref_parse_sample_list.argtypes = [C.c_char_p, _P, C.c_size_t,
+                                                  C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+        L.eimp_qref_mask_sample_list.argtypes = [_P, C.c_char_p, C.POINTER(QrefInfo)]
+        L.eimp_qref_read_sites_af.argtypes = [_P, C.c_size_t, C.c_size_t, _P, C.c_size_t, _P, _P,
+                                              C.c_int]
         L._qref_bound = True
     return L
 
@@ -106,3 +112,35 @@ class Qref:
         rows = np.zeros((n, rw), np.uint64)
         _check(_bind().eimp_qref_read_sites(self._h, m0, n, _p(rows), rw, threads))
         return rows
+
+    def mask(self, samples) -> None:
+        """Masks samples (strictly ascending indices of the file; empty clears
+        the mask): rows, counts and haploid flags then cover the kept ones."""
+        s = np.ascontiguousarray(samples, np.uint64)
+        _check(_bind().eimp_qref_set_mask(self._h, _p(s) if s.size else None, s.size,
+                                          C.byref(self.info)))
+
+    def mask_sample_list(self, path: str) -> None:
+        """Masks the withdrawals of a sample list (VCFData::parseSampleList)."""
+        _check(_bind().eimp_qref_mask_sample_list(self._h, str(path).encode(),
+                                                  C.byref(self.info)))
+
+    def allele_freqs(self, m0: int = 0, n: int | None = None, threads: int = 0):
+        """(allele frequency after masking, masked variant alleles) of
+        variants [m0, m0+n)."""
+        n = self.Mref - m0 if n is None else n
+        af = np.zeros(max(n, 1), np.float32)
+        cnt = np.zeros(max(n, 1), np.uint64)
+        _check(_bind().eimp_qref_read_sites_af(self._h, m0, n, None, 0, _p(af), _p(cnt), threads))
+        return af[:n], cnt[:n]
+
+
+def parse_sample_list(path: str):
+    """(withdrawal sample indices, samples listed) of a sample list."""
+    L = _bind()
+    nm, ns = C.c_size_t(), C.c_size_t()
+    _check(L.eimp_qref_parse_sample_list(str(path).encode(), None, 0, C.byref(nm), C.byref(ns)))
+    out = np.zeros(max(nm.value, 1), np.uint64)
+    _check(L.eimp_qref_parse_sample_list(str(path).encode(), _p(out), nm.value, C.byref(nm),
+                                         C.byref(ns)))
+    return out[:nm.value], ns.value

@@ -382,8 +382,8 @@ int eimp_phase_batch(const eimp_phase_params *params, size_t n, const eimp_phase
 /* panel format (qref_spec.md; VCFData::qRefOpenReadMeta and                 */
 /* qRefLoadNextVariant, src/VCFData.cpp:2107-2432; run-length code           */
 /* src/utils.h:133-315) into the site-major rows eimp_engine_initialize      */
-/* takes.  Sample masking and target matching (VCF/HTSlib) are not part of   */
-/* it.                                                                       */
+/* takes, with the reference's sample masking of participant withdrawals.    */
+/* Target matching (VCF/HTSlib) is not part of it.                           */
 /* ------------------------------------------------------------------------ */
 typedef struct eimp_qref eimp_qref;
 
@@ -402,9 +402,11 @@ typedef struct eimp_qref_info {
 int eimp_qref_open(const char *path, eimp_qref **qref, eimp_qref_info *info);
 int eimp_qref_close(eimp_qref *qref);
 
-/* Any pointer may be NULL.  positions: Mref int64 (0-based), af: Mref float,
- * haploid: Nref bytes (1 = haploid; all equal when the file stores no
- * flags), ma_flags: ceil(Mref/64) words (bit m = split from a multi-allelic). */
+/* Any pointer may be NULL.  positions: Mref int64 (0-based), af: Mref float
+ * as stored in the file (the masked frequencies come from
+ * eimp_qref_read_sites_af), haploid: Nref bytes of the kept samples (1 =
+ * haploid; all equal when the file stores no flags), ma_flags: ceil(Mref/64)
+ * words (bit m = split from a multi-allelic). */
 int eimp_qref_metadata(eimp_qref *qref, int64_t *positions, float *af, uint8_t *haploid,
                        uint64_t *ma_flags);
 
@@ -418,6 +420,34 @@ int eimp_qref_variant(eimp_qref *qref, size_t m, const char **ref, const char **
  * 2*Nref zero), decoded over `threads` host threads (<= 0: all). */
 int eimp_qref_read_sites(eimp_qref *qref, size_t m0, size_t n, uint64_t *rows, size_t row_words,
                          int threads);
+
+/* Masks samples (participant withdrawals; replaces the mask the reference
+ * builds in VCFData::parseSampleList, src/VCFData.cpp:3698-3746, and applies
+ * in qRefOpenReadMeta / qRefLoadNextVariant, src/VCFData.cpp:2139-2208 and
+ * 2418-2530): `samples` strictly ascending sample indices of the file (n = 0
+ * clears the mask).  Afterwards Nref, NrefHap and hap_words describe the kept
+ * samples (*info, may be NULL, receives them), rows hold only their
+ * haplotypes and eimp_qref_metadata's haploid flags only theirs. */
+int eimp_qref_set_mask(eimp_qref *qref, const uint64_t *samples, size_t n, eimp_qref_info *info);
+
+/* VCFData::parseSampleList (src/VCFData.cpp:3698-3746): one sample per line
+ * (first token; the second one after two header lines in a file ending in
+ * ".sample"), a withdrawal is an ID of 'W' or '-' followed by digits only.
+ * Writes up to cap withdrawal indices (masked may be NULL to count),
+ * *nmasked = withdrawals, *nsamples = samples listed. */
+int eimp_qref_parse_sample_list(const char *path, uint64_t *masked, size_t cap, size_t *nmasked,
+                                size_t *nsamples);
+
+/* Masks the withdrawals of a sample list; the list must name as many samples
+ * as the file holds when it has any withdrawal (src/VCFData.cpp:2139-2142). */
+int eimp_qref_mask_sample_list(eimp_qref *qref, const char *path, eimp_qref_info *info);
+
+/* eimp_qref_read_sites plus, per variant, the allele frequency the reference
+ * holds after masking (af, src/VCFData.cpp:2426-2432 and 2524-2529: the
+ * file's frequency without a mask) and the variant alleles the masked
+ * samples carried (masked).  rows, af and masked may each be NULL. */
+int eimp_qref_read_sites_af(eimp_qref *qref, size_t m0, size_t n, uint64_t *rows,
+                            size_t row_words, float *af, uint64_t *masked, int threads);
 
 #ifdef __cplusplus
 }

@@ -110,6 +110,8 @@ def ref():
         R.ref_rle_encode.argtypes = [_P, _U64, _P, _U64]
         R.ref_rle_decode.restype = C.c_int
         R.ref_rle_decode.argtypes = [_P, _U64, _P, _U64]
+        R.ref_rle_decode_masked.restype = C.c_int
+        R.ref_rle_decode_masked.argtypes = [_P, _U64, _P, _U64, _P, _P, _U64, _P]
         _REF = R
     return _REF
 
@@ -349,6 +351,19 @@ def ref_rle_decode(code: bytes, nwords: int):
     out = np.zeros(nwords, np.uint64)
     ref().ref_rle_decode(_p(buf), len(code), _p(out), nwords)
     return out
+
+
+def ref_rle_decode_masked(code: bytes, nwords: int, masked, masked_haploid):
+    """The reference's Qref run-length decoder with sample masking: (words,
+    count of the masked samples' variant alleles)."""
+    buf = np.frombuffer(code, np.uint8).copy() if code else np.zeros(1, np.uint8)
+    ms = np.ascontiguousarray(masked, np.uint64)
+    mh = np.ascontiguousarray(masked_haploid, np.uint8)
+    out = np.zeros(max(nwords, 1), np.uint64)
+    cnt = np.zeros(1, np.uint64)
+    ref().ref_rle_decode_masked(_p(buf), len(code), _p(ms) if ms.size else None, ms.size,
+                                _p(mh) if mh.size else None, _p(out), nwords, _p(cnt))
+    return out[:nwords], int(cnt[0])
 
 
 def build_ref_pbwt(rows, K, absperm=True):

@@ -180,6 +180,23 @@ int ref_rle_decode(const uint8_t *code, uint64_t n, uint64_t *words, uint64_t nw
     return 0;
 }
 
+// Its decoder with sample masking (src/utils.h:193-315): the haplotypes of
+// the masked samples (ascending sample indices) are dropped, their variant
+// alleles counted into *masked_alleles (a haploid masked sample counts once).
+int ref_rle_decode_masked(const uint8_t *code, uint64_t n, const uint64_t *masked, uint64_t nmasked,
+                          const uint8_t *masked_haploid, uint64_t *words, uint64_t nwords,
+                          uint64_t *masked_alleles) {
+    std::vector<char> enc(code, code + n);
+    std::memset(words, 0, nwords * sizeof(uint64_t));
+    std::vector<size_t> ms(masked, masked + nmasked);
+    std::vector<bool> mh(nmasked);
+    for (uint64_t i = 0; i < nmasked; i++) mh[i] = masked_haploid[i] != 0;
+    size_t cnt = 0;
+    runlengthDecode(enc, words, nwords, ms, mh, cnt);
+    *masked_alleles = cnt;
+    return 0;
+}
+
 // Target::phaseExtPBWT (src/Target.cpp:291-777) restated around the
 // reference's own Beam over PBWT(cpbwt, count0, K, row_stride, Mp)
 // (src/GPUHandler.cpp:259-260).  Target.cpp itself needs VCFData (HTSlib),

@@ -17,6 +17,10 @@ it writes are plain data (inputs and the reference's outputs) in .npz files:
   ref_rle.npz       haplotype bit vectors -> the reference's Qref run-length
                     code (src/utils.h:133-190) or its refusal (code not
                     shorter than the raw vector)
+  ref_rle_masked.npz  run-length coded haplotype records, haploid flags and
+                    masked samples -> the reference's decoder with sample
+                    masking (src/utils.h:193-315): the kept haplotypes and
+                    the count of the masked samples' variant alleles
   ref_phase.npz     the reference's beam search (src/Beam.cpp,
                     src/BeamPath.cpp, src/HaplotypePath.cpp) driven as
                     Target::phaseExtPBWT (src/Target.cpp:291-777) on the
@@ -24,7 +28,8 @@ it writes are plain data (inputs and the reference's outputs) in .npz files:
                     dosages, call confidences and totals, with a hash of the
                     inputs (regenerated from their seeds by the test)
 
-Usage:  python tests/golden/make_golden.py
+Usage:  python tests/golden/make_golden.py [fixture ...]   (default: all;
+        a fixture is named by its file without "ref_" and ".npz")
 """
 import os
 import sys
@@ -201,15 +206,81 @@ def rle_fixtures():
     np.savez_compressed(os.path.join(HERE, "ref_rle.npz"), **out)
 
 
+def rle_masked_cases(rng):
+    """(samples, haploid flags, record bits, masked samples): masks at the
+    first and last sample, across word and 512-bit boundaries, runs of
+    neighbours, all but one and all samples; haploid samples stored
+    homozygous and, a few, heterozygous."""
+    cases = []
+    for t in range(40):
+        N = int(rng.choice([1, 2, 31, 32, 33, 200, 256, 257, 700]))
+        p = float(rng.choice([0.01, 0.05, 0.95, 0.99]))
+        bits = (rng.random(2 * N) < p).astype(np.uint8)
+        if t % 3 == 0:  # runs of samples sharing an allele
+            bits = np.repeat(rng.random(2 * N // 8 + 1) < 0.5, 8)[:2 * N].astype(np.uint8)
+        hap = (rng.random(N) < float(rng.choice([0.0, 0.2, 1.0]))).astype(np.uint8)
+        for s in np.flatnonzero(hap):
+            if rng.random() < 0.85:
+                bits[2 * s + 1] = bits[2 * s]
+        kind = t % 8
+        if kind == 0:
+            ms = [0]
+        elif kind == 1:
+            ms = [N - 1]
+        elif kind == 2:
+            ms = [s for s in (15, 16, 31, 32, 255, 256) if s < N]
+        elif kind == 3:
+            a = int(rng.integers(0, N))
+            ms = list(range(a, min(N, a + int(rng.integers(1, 40)))))
+        elif kind == 4:
+            keep = int(rng.integers(0, N))
+            ms = [s for s in range(N) if s != keep]
+        elif kind == 5:
+            ms = list(range(N))
+        else:
+            ms = sorted(rng.choice(N, int(rng.integers(0, N + 1)), replace=False).tolist())
+        cases.append((N, hap, bits, np.asarray(sorted(set(ms)), np.uint64)))
+    return cases
+
+
+def rle_masked_fixtures():
+    rng = np.random.default_rng(91)
+    out = {}
+    n = 0
+    for N, hap, bits, ms in rle_masked_cases(rng):
+        hw = (2 * N + 511) // 512 * 8
+        full = np.zeros(hw * 64, np.uint8)
+        full[:2 * N] = bits
+        w = np.packbits(full, bitorder="little").view(np.uint64)
+        code = oracle.ref_rle_encode(w)
+        if code is None:
+            continue
+        Nk = N - ms.size
+        nwk = (2 * Nk + 511) // 512 * 8
+        got, cnt = oracle.ref_rle_decode_masked(code, nwk, ms, hap[ms.astype(np.int64)])
+        out[f"c{n}_N"] = np.int64(N)
+        out[f"c{n}_haploid"] = hap
+        out[f"c{n}_words"] = w
+        out[f"c{n}_code"] = np.frombuffer(code, np.uint8)
+        out[f"c{n}_mask"] = ms
+        out[f"c{n}_rows"] = got
+        out[f"c{n}_masked_alleles"] = np.int64(cnt)
+        n += 1
+    out["ncases"] = np.int64(n)
+    np.savez_compressed(os.path.join(HERE, "ref_rle_masked.npz"), **out)
+
+
+FIXTURES = {"pbwt": pbwt_fixtures, "genotype": genotype_fixtures,
+            "panel_pbwt": panel_pbwt_fixtures, "phase": phase_fixtures, "rle": rle_fixtures,
+            "rle_masked": rle_masked_fixtures}
+
 if __name__ == "__main__":
     oracle.build()
     if not oracle.ref_available():
         sys.exit("reference harness not built (is /root/reference present?)")
-    pbwt_fixtures()
-    genotype_fixtures()
-    panel_pbwt_fixtures()
-    phase_fixtures()
-    rle_fixtures()
-    for f in ("ref_pbwt.npz", "ref_genotype.npz", "ref_panel_pbwt.npz", "ref_phase.npz",
-              "ref_rle.npz"):
+    names = sys.argv[1:] or list(FIXTURES)
+    for name in names:
+        FIXTURES[name]()
+    for name in names:
+        f = f"ref_{name}.npz"
         print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")
