@@ -270,9 +270,10 @@ def phase_bench(args):
     print(json.dumps(out), flush=True)
 
 
-def find_traffic(kernel_name, workload, targets_per_step):
+def find_traffic(kernel_name, workload, targets_per_step, layout="gpu"):
     """HBM bytes per launch of kernel_name measured with rocprofv3 counter
-    passes (FETCH_SIZE, WRITE_SIZE) on this workload and submission size."""
+    passes (FETCH_SIZE, WRITE_SIZE) on this workload, submission size and
+    output layout."""
     import glob
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*traffic*.json"))):
         try:
@@ -280,21 +281,24 @@ def find_traffic(kernel_name, workload, targets_per_step):
         except (OSError, ValueError):
             continue
         if (tr.get("kernel") == kernel_name and tr.get("workload") == workload
-                and tr.get("targets_per_step") == targets_per_step):
+                and tr.get("targets_per_step") == targets_per_step
+                and tr.get("layout", "gpu") == layout):
             return round(float(tr["traffic_bytes_per_launch"])), os.path.relpath(path, ROOT)
     return None, None
 
 
-def find_sq(kernel_name, workload):
+def find_sq(kernel_name, workload, layout="gpu"):
     """Issue counters (SQ_INSTS_VALU etc. per wave-step) of kernel_name on this
-    workload from a committed counter summary, if one was taken."""
+    workload and output layout from a committed counter summary, if one was
+    taken."""
     import glob
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*sq*.json"))):
         try:
             sq = json.load(open(path))
         except (OSError, ValueError):
             continue
-        if sq.get("kernel", "").startswith(kernel_name) and sq.get("workload") == workload:
+        if (sq.get("kernel", "").startswith(kernel_name) and sq.get("workload") == workload
+                and sq.get("layout", "gpu") == layout):
             return dict(sq, source=os.path.relpath(path, ROOT))
     return None
 
@@ -343,6 +347,10 @@ def main():
                     help="stream contexts (device-driven, queue kernels) or the synchronous "
                          "batch engine")
     ap.add_argument("--contexts", type=int, default=2, help="stream contexts in flight")
+    ap.add_argument("--layout", choices=["gpu", "compact"], default="gpu",
+                    help="output rows held on the device: the reference GPU engine's drop-in "
+                         "rows (4*Kwords words) or the CPU layout of src/PBWT.cpp "
+                         "(2*ceil(K/32) words)")
     ap.add_argument("--tune-batches", type=int, nargs="+", default=[256, 384, 512, 1024],
                     help="targets-per-submission candidates of the autotuner")
     ap.add_argument("--tune", action=argparse.BooleanOptionalAction, default=True,
@@ -467,7 +475,8 @@ def batch_bench(args) -> dict:
             from eagleimp_amd.tune import tune
             nt = min(T, max(args.tune_batches))
             cands = [b for b in args.tune_batches if b <= nt] or [nt]
-            tuned = tune(eng, tg, list(range(nt)), batches=cands, use_cache=False, log=log)
+            tuned = tune(eng, tg, list(range(nt)), batches=cands, use_cache=False, log=log,
+                         layout=args.layout)
             if not args.batch:
                 batch = tuned["batch"]
             log(f"autotuner: variant {tuned['variant']} ({tuned['threads']} threads x "
@@ -529,7 +538,7 @@ def batch_bench(args) -> dict:
 
         def contexts(timed):
             return [eng.stream(batch, maxW, maxK, pool, graph=not timed, timed=timed,
-                               digest=False)
+                               digest=False, compact=args.layout == "compact")
                     for _ in range(nctx)]
 
         sts = contexts(timed=True)
@@ -593,8 +602,8 @@ def batch_bench(args) -> dict:
     # reference read by each direction
     kbytes_launch = float(pbwt_alg_bytes(Mp_all, K_all).sum()) / max(pbwt_launches, 1)
     kernel_name = "k_pbwt" if args.engine == "batch" else "k_pbwt_persist"
-    traffic, traffic_src = find_traffic(kernel_name, cfg["name"], batch)
-    sq = find_sq(kernel_name, cfg["name"])
+    traffic, traffic_src = find_traffic(kernel_name, cfg["name"], batch, args.layout)
+    sq = find_sq(kernel_name, cfg["name"], args.layout)
 
     # --- CPU baseline (oracle, OpenMP over targets) on a bounded sample; its
     # targets are those whose timed outputs are still on the device, and the
@@ -690,6 +699,7 @@ def batch_bench(args) -> dict:
             "targets_per_step": batch, "K": sorted(set(int(k) for k in K_all)),
             "window": "full" if cfg.get("wmin") is None else "log-uniform",
             "mean_Mp": round(float(Mp_all.mean()), 1), "contexts": nctx,
+            "layout": args.layout if args.engine == "stream" else "gpu",
         },
         "bit_exact": bit_exact,
         "verified_targets": verified,

@@ -67,10 +67,13 @@ def candidates(Kmax: int):
 
 
 def tune(eng: Engine, targets, idx, batches=(128, 256), reserve_bytes: int = 24 << 30,
-         use_cache: bool = True, log=None) -> dict:
+         use_cache: bool = True, log=None, layout: str = "gpu") -> dict:
     """Times every (variant, batch) candidate on the sample targets[idx] and
     sets the fastest variant on the engine.  Returns the choice and the table
-    of per-target device milliseconds."""
+    of per-target device milliseconds.  layout: the contexts' output rows,
+    "gpu" (drop-in) or "compact"."""
+    if layout not in ("gpu", "compact"):
+        raise ValueError(f"layout {layout!r}: 'gpu' or 'compact'")
     idx = list(idx)
     Kmax = int(min(int(np.max(targets.K[idx])), eng.Ncorr))
     Wmax = int(np.max(targets.W[idx]))
@@ -78,7 +81,7 @@ def tune(eng: Engine, targets, idx, batches=(128, 256), reserve_bytes: int = 24 
     if not vs:
         raise ValueError(f"no PBWT variant holds K = {Kmax}")
     batches = [b for b in batches if b <= len(idx)] or [len(idx)]
-    key = _key(eng, Kmax, Wmax, batches, vs)
+    key = _key(eng, Kmax, Wmax, batches, vs) + ("" if layout == "gpu" else "/" + layout)
     db = _load() if use_cache else {}
     if key in db:
         best = db[key]
@@ -97,7 +100,8 @@ def tune(eng: Engine, targets, idx, batches=(128, 256), reserve_bytes: int = 24 
             if b in seen:
                 continue
             seen.add(b)
-            st = eng.stream(b, Wmax, Kmax, pool, timed=True, graph=False, digest=False)
+            st = eng.stream(b, Wmax, Kmax, pool, timed=True, graph=False, digest=False,
+                            compact=layout == "compact")
             sub = idx[:b]
             t0 = time.perf_counter()
             st.submit([targets.gt[i] for i in sub], targets.s0[sub], targets.W[sub],
@@ -113,7 +117,8 @@ def tune(eng: Engine, targets, idx, batches=(128, 256), reserve_bytes: int = 24 
                        chunks_per_wave=v["chunks_per_wave"], batch=b, targets=done,
                        ms_per_target=dev / max(done, 1), pbwt_ms=ms["pbwt"], cref_ms=ms["cref"],
                        wall_s=round(wall, 3),
-                       bytes_per_target=float(region_bytes(res.Mp[fit], res.K[fit]).mean())
+                       bytes_per_target=float(region_bytes(res.Mp[fit], res.K[fit],
+                                                           layout).mean())
                        if done else 0.0, deferred=b - done)
             table.append(row)
             if log:

@@ -1,5 +1,5 @@
 // eimp_gpu_handler.hpp -- C++ drop-in for the reference's GPUHandler
-// (src/GPUHandler.h:45-88, src/GPUHandler.cpp) over the C ABI in eimp.h.
+// (src/GPUHandler.h:47-90, src/GPUHandler.cpp) over the C ABI in eimp.h.
 //
 // The reference's producer runs Target::prepPhase (findBestHaps, findMonos,
 // findCallLocs) on the host under OpenMP and hands only the condensed
