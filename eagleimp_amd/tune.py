@@ -101,7 +101,7 @@ def tune(eng: Engine, targets, idx, batches=(128, 256), reserve_bytes: int = 24 
                 continue
             seen.add(b)
             st = eng.stream(b, Wmax, Kmax, pool, timed=True, graph=False, digest=False,
-                            compact=layout == "compact")
+                            **({"compact": True} if layout == "compact" else {}))
             sub = idx[:b]
             t0 = time.perf_counter()
             st.submit([targets.gt[i] for i in sub], targets.s0[sub], targets.W[sub],
