@@ -947,10 +947,6 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
     uint16_t *stage = (uint16_t *)smem;
     uint32_t *wzero = smem + NPOS;                 // NW: zeros of each wave's chunks
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = lane_id();
-    // 4 * lane, opaque to the compiler: the uniform-chunk path adds it to a
-    // scalar byte base (one VALU instruction) instead of scaling after the join
-    uint32_t ln4;
-    asm("" : "=v"(ln4) : "0"(ln * 4u));
     const uint32_t ch0 = wave * NS;                // first chunk of this wave
     const gptr_t<const uint16_t> cref = gmem((const uint16_t *)T.cref);
     // wave-uniform flags as scalars: their branches stay scalar branches
@@ -1079,29 +1075,23 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
         const uint32_t zbv = run + zex;
         // pass 2: stable partition, zeros first.  run / ob: positions of the
         // zeros / ones of the row before the current chunk (ones go after all
-        // cnt0 zeros).  A chunk whose 64 bits are equal moves as one block
-        // (its lanes in order); otherwise my bit is bit ln of the ballot
-        // (inverse ballot: the mask itself selects), ones go to ob + (ones
-        // below me), zeros to run + (zeros below me) = run + |ln - t|.
+        // cnt0 zeros).  My bit is bit ln of the ballot (inverse ballot: the
+        // mask itself selects); ones go to ob + (ones below me), zeros to
+        // run + (zeros below me) = run + |ln - t|.  No branch on the chunk's
+        // contents: the step is bound by the instructions its waves issue,
+        // and a test for all-equal chunks costs more (scalar compare and
+        // taken branches) than the vector instructions it would skip.
         {
             uint32_t ob = cnt0 + ch0 * 64u - run;
             static_for<0, NS>([&](auto ci) {
                 constexpr int c = decltype(ci)::value;
                 const uint64_t m = mk[c];
                 const uint32_t ones = (uint32_t)__popcll(m);
-                uint32_t addr;  // LDS byte address of my destination in pay[]
-                if ((ones & 63u) == 0u) {  // all zeros or all ones: lanes in order
-                    // one vector add of the scalar block base; written as asm so
-                    // that the compiler does not merge it with the other path's
-                    // address arithmetic after the join
-                    const uint32_t base = (ones ? ob : run) * 4u;
-                    asm("v_add_u32 %0, %1, %2" : "=v"(addr) : "s"(base), "v"(ln4));
-                } else {
-                    const uint32_t t = mbcnt(m);  // ones below me
-                    const uint32_t od = ob + t;
-                    const uint32_t zd = __builtin_amdgcn_sad_u16(ln, t, run);
-                    addr = (__builtin_amdgcn_inverse_ballot_w64(m) ? od : zd) * 4u;
-                }
+                const uint32_t t = mbcnt(m);  // ones below me
+                const uint32_t od = ob + t;
+                const uint32_t zd = __builtin_amdgcn_sad_u16(ln, t, run);
+                // LDS byte address of my destination in pay[]
+                const uint32_t addr = (__builtin_amdgcn_inverse_ballot_w64(m) ? od : zd) * 4u;
                 *(lds_ptr_t<uint32_t>)(size_t)addr = p[c];
                 run += 64u - ones;
                 ob += ones;
