@@ -450,6 +450,7 @@ def batch_bench(args) -> dict:
 
     graph_tps = None
     tuned = None
+    stored = None      # (target, its stored outputs) copied out of a timed submission
     nctx = None
     post = {}          # target -> (Mp, digest pbwt, digest count0) of its stored timed outputs
     pass_tps = []
@@ -567,6 +568,9 @@ def batch_bench(args) -> dict:
                 dp, dc = st.digest_outputs()
                 for j, t in enumerate(b):
                     if int(res.Mp[j]) > 0:
+                        if not post and not args.no_cpu:
+                            # the first of them is also compared byte for byte
+                            stored = (t, st.copy_target(j))
                         post[t] = (int(res.Mp[j]), int(dp[j]), int(dc[j]))
             st.close()
         tim = {k: (v, 0) for k, v in kms.items()}
@@ -610,6 +614,7 @@ def batch_bench(args) -> dict:
     # oracle's digests are compared with the digests of those outputs
     cpu = None
     verified = 0
+    rows_compared, rows_ok = 0, True
     bit_exact = None
     mismatches = []
     if not args.no_cpu:
@@ -619,9 +624,22 @@ def batch_bench(args) -> dict:
         pool_ids = list(post) if post else [i % T for i in range(T)]
         calib = pool_ids[0]
         t = time.perf_counter()
-        r1 = oracle.process_batch(pv, [tg.gt[calib]], [tg.s0[calib]], [tg.W[calib]],
-                                  [tg.K[calib]], nthreads=1)
+        full = stored is not None and stored[0] == calib
+        r1 = oracle.process_target(pv, tg.gt[calib], tg.s0[calib], tg.W[calib], tg.K[calib],
+                                   keep=full, rows=full)
         one = time.perf_counter() - t
+        if full:
+            # rows and count0 of the calibration target as stored by the timed
+            # run, against the oracle's, word for word (drop-in rows: the
+            # logical words, then zero tail pairs)
+            got, kw = stored[1], oracle.pbwt_row_words(int(tg.K[calib]))
+            rows_ok = (got["pbwt"].shape[0] == r1["pbwt"].shape[0]
+                       and np.array_equal(got["pbwt"][:, :kw], r1["pbwt"])
+                       and not got["pbwt"][:, kw:].any()
+                       and np.array_equal(got["count0"], r1["count0"]))
+            rows_compared = 1
+            log(f"stored rows of target {calib} vs the oracle's, word for word: {rows_ok}")
+        r1 = {"Mp": [r1["Mp"]], "dig_pbwt": [r1["dig_pbwt"]], "dig_count0": [r1["dig_count0"]]}
         n = int(max(1, cores * max(1, round(args.cpu_seconds / max(one, 1e-3)))))
         ids = pool_ids[1:1 + n] if len(pool_ids) > 1 else pool_ids
         n = len(ids)
@@ -646,6 +664,7 @@ def batch_bench(args) -> dict:
                 if not ok:
                     bit_exact = False
                     mismatches.append(int(tt))
+            bit_exact = bit_exact and rows_ok
             log(f"timed outputs of {verified} targets vs the oracle: bit_exact={bit_exact}"
                 + (f" (mismatch: {mismatches[:8]})" if mismatches else ""))
 
@@ -703,9 +722,11 @@ def batch_bench(args) -> dict:
         },
         "bit_exact": bit_exact,
         "verified_targets": verified,
+        "rows_compared": rows_compared,
         "verified_how": "digests of the timed submissions' own stored outputs (rows + count0, "
                         "recomputed on the device after the timed region) vs the oracle's, for "
-                        "every target of the CPU-baseline sample" if args.engine == "stream"
+                        "every target of the CPU-baseline sample; rows_compared of them also "
+                        "copied out and compared word for word" if args.engine == "stream"
                         else "resubmission with the device digest",
         "b_alg_GBps": round(float(balg_t.sum()) / wall / 1e9, 2),
         "b_alg_frac": round(float(balg_t.sum()) / wall / 1e9 / HBM_PEAK_GBS, 4),
@@ -725,9 +746,11 @@ def batch_bench(args) -> dict:
             "traffic_over_alg": None if traffic is None else round(traffic / balg_launch, 3),
             "traffic_over_kernel_bytes": None if traffic is None else round(traffic / kbytes_launch, 3),
             "limiter": None if sq is None else (
-                f"step latency (VALU busy {100 * sq['valu_busy']:.1f}%, "
-                f"{sq['valu_per_wave_step']} VALU / {sq['salu_per_wave_step']} SALU / "
-                f"{sq['lds_per_wave_step']} LDS instructions per wave-step)"),
+                f"instruction issue: {sq['valu_per_wave_step']} VALU + "
+                f"{sq['salu_per_wave_step']} SALU + {sq['lds_per_wave_step']} LDS instructions "
+                f"per wave-step, a step takes {sq.get('issue_model_ratio', float('nan'))} x "
+                f"issuing them one at a time at 4 cycles each; VALU busy "
+                f"{100 * sq['valu_busy']:.1f}%"),
             "sq": None if sq is None else {k: sq[k] for k in sq if k not in ("kernel", "workload")},
         },
         "kernel_ms": {k: round(v[0], 3) for k, v in tim.items()},
