@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -834,36 +835,58 @@ int run_batch_impl(eimp_engine *e, size_t n, const eimp_target *targets, float c
 
 // ---------------------------------------------------------------------------
 // Stream context: one HIP stream, fixed-capacity buffers, and the batch's
-// launch sequence captured once as a hipGraph.  A submission copies the
-// batch header, the target table and the genotypes (one pinned staging
-// buffer), replays the graph, and copies back the split counts, the table
+// launch sequence captured once as two hipGraphs: the preparation (distances,
+// top-K, monos, call locations; the context's own buffers) and the build
+// (planner, condensed reference, PBWT; the output pool).  A submission copies
+// the batch header, the target table and the genotypes (one pinned staging
+// buffer), replays the graphs, and copies back the split counts, the table
 // (output placement) and the digests.  Nothing in the sequence depends on
 // host-side sizes: kernels read the batch size from the header and the
 // planner sizes the outputs on the device.
+//
+// Contexts may share one output pool (eimp_stream_create_shared): each
+// build then waits for the pool's previous build, whichever context ran it,
+// while the preparation of a submission runs beside the other context's
+// build.  A context's pool outputs stay valid until a context sharing the
+// pool submits again.
 // ---------------------------------------------------------------------------
+struct StreamPool {
+    DevBuf buf;
+    uint64_t cap = 0;
+    uint64_t seq = 0;                 // builds submitted into the pool
+    hipEvent_t last_build = nullptr;  // the last submitted build's event (its context owns it)
+};
+
 struct eimp_stream {
     eimp_engine *e = nullptr;
     hipStream_t stream = nullptr;
     uint32_t nmax = 0, Wmax = 0, Kmax = 0, flags = 0, max_ww = 0;
-    uint64_t pool_cap = 0, gen = 0;
+    uint64_t gen = 0;
     bool timed = false;
     hipEvent_t kev[EIMP_NKCLASS + 1] = {};  // timed contexts: events between kernel classes
+    hipEvent_t kev_prep_end = nullptr;      // timed contexts: end of the preparation
     double kms[EIMP_NKCLASS] = {};
     // device buffers; d_in = header (256 B) | TgtDev[nmax] | genotypes
     DevBuf d_in, d_nerr, d_bh, d_bhflags, d_mono, d_splitsites, d_splits, d_cMdiffs, d_stats,
-        d_digest, d_blockpref, d_pitems, d_pool, d_tmp;
+        d_digest, d_blockpref, d_pitems, d_tmp;
+    std::shared_ptr<StreamPool> pool;  // the output pool, maybe shared with other contexts
     uint64_t tab_off = 0, gt_off = 0, gt_cap_words = 0;
     HostBuf h_in, h_out;  // h_out = stats | digests | TgtDev table
     // launch configuration
     int pbwt_v = -1;
     size_t pbwt_lds = 0;
     uint32_t cref_grid[2] = {0, 0}, pbwt_grid = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph_prep = nullptr, graph_build = nullptr;
+    hipGraphExec_t exec_prep = nullptr, exec_build = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t build_done = nullptr;
     // submission state
     bool pending = false, collected = false;
     uint32_t n_last = 0;
+    uint64_t build_seq = 0;  // pool->seq of this context's last build
+
+    // the pool still holds this context's last outputs
+    bool outputs_current() const { return pool && build_seq == pool->seq; }
 
     BatchDev *d_batch() const { return d_in.as<BatchDev>(); }
     TgtDev *d_tg() const { return reinterpret_cast<TgtDev *>(d_in.as<char>() + tab_off); }
@@ -874,12 +897,28 @@ struct eimp_stream {
 
 namespace {
 
-int stream_launch_sequence(eimp_stream *s) {
+const char *const kPoolReused =
+    "the output pool holds a later submission of a context sharing it";
+
+// the preparation: a3..a6 in the context's own buffers
+int stream_launch_prep(eimp_stream *s) {
+    eimp_engine *e = s->e;
+    const uint32_t *d_n = &s->d_batch()->n;
+    for (int cls = EIMP_KCLASS_DISTANCE; cls <= EIMP_KCLASS_SPLITS; cls++) {
+        if (s->timed) HIPCHK(hipEventRecord(s->kev[cls], s->stream));
+        HIPCHK(launch_prep_dev(cls, e->pd, s->d_tg(), d_n, s->nmax, s->max_ww, s->stream));
+    }
+    if (s->timed) HIPCHK(hipEventRecord(s->kev_prep_end, s->stream));
+    return EIMP_OK;
+}
+
+// the build: planner, a7, a9 (and the count0 digest) into the output pool
+int stream_launch_build(eimp_stream *s) {
     eimp_engine *e = s->e;
     const uint32_t *d_n = &s->d_batch()->n;
     PlanArgs A{};
-    A.pool = s->d_pool.as<char>();
-    A.pool_cap = s->pool_cap;
+    A.pool = s->pool->buf.as<char>();
+    A.pool_cap = s->pool->cap;
     A.dig_base = s->d_digest.as<uint64_t>();
     A.blockpref = s->d_blockpref.as<uint32_t>();
     A.blockpref_id = s->d_blockpref.as<uint32_t>() + s->nmax + 1;
@@ -893,10 +932,6 @@ int stream_launch_sequence(eimp_stream *s) {
         if (s->timed) HIPCHK(hipEventRecord(s->kev[cls], s->stream));
         return EIMP_OK;
     };
-    for (int cls = EIMP_KCLASS_DISTANCE; cls <= EIMP_KCLASS_SPLITS; cls++) {
-        CHK(mark(cls));
-        HIPCHK(launch_prep_dev(cls, e->pd, s->d_tg(), d_n, s->nmax, s->max_ww, s->stream));
-    }
     CHK(mark(EIMP_KCLASS_CREF));
     HIPCHK(launch_plan(s->d_tg(), s->d_batch(), A, s->stream));
     HIPCHK(launch_cref_persist(e->pd, s->d_tg(), s->d_batch(), A.blockpref, A.blockpref_id, s->Kmax,
@@ -911,27 +946,58 @@ int stream_launch_sequence(eimp_stream *s) {
 
 void stream_release(eimp_stream *s) {
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    if (s->exec) (void)hipGraphExecDestroy(s->exec);
-    if (s->graph) (void)hipGraphDestroy(s->graph);
+    if (s->pool) {
+        // the build event dies with this context: later builds need not wait for it
+        if (s->build_done && s->pool->last_build == s->build_done) s->pool->last_build = nullptr;
+        if (s->pool.use_count() == 1) s->pool->buf.release();
+        s->pool.reset();
+    }
+    for (hipGraphExec_t x : {s->exec_prep, s->exec_build})
+        if (x) (void)hipGraphExecDestroy(x);
+    for (hipGraph_t g : {s->graph_prep, s->graph_build})
+        if (g) (void)hipGraphDestroy(g);
     for (auto &ev : s->ev)
         if (ev) (void)hipEventDestroy(ev);
     for (auto &ev : s->kev)
         if (ev) (void)hipEventDestroy(ev);
+    if (s->kev_prep_end) (void)hipEventDestroy(s->kev_prep_end);
+    if (s->build_done) (void)hipEventDestroy(s->build_done);
     DevBuf *bufs[] = {&s->d_in, &s->d_nerr, &s->d_bh, &s->d_bhflags, &s->d_mono, &s->d_splitsites,
                       &s->d_splits, &s->d_cMdiffs, &s->d_stats, &s->d_digest, &s->d_blockpref,
-                      &s->d_pitems, &s->d_pool, &s->d_tmp};
+                      &s->d_pitems, &s->d_tmp};
     for (auto *b : bufs) b->release();
     s->h_in.release();
     s->h_out.release();
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
-int stream_setup(eimp_stream *s, size_t pool_bytes) {
+// capture one part of the launch sequence as a graph
+int stream_capture(eimp_stream *s, int (*part)(eimp_stream *), hipGraph_t *graph,
+                   hipGraphExec_t *exec) {
+    HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = part(s);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s->stream, &g);
+    if (rc != EIMP_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ec != hipSuccess) return fail(EIMP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+    *graph = g;
+    HIPCHK(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+    return EIMP_OK;
+}
+
+// pool: an existing pool to share, or null to allocate one of pool_bytes
+int stream_setup(eimp_stream *s, size_t pool_bytes, const std::shared_ptr<StreamPool> &pool) {
     eimp_engine *e = s->e;
     HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     for (auto &ev : s->ev) HIPCHK(hipEventCreate(&ev));
-    if (s->timed)
+    HIPCHK(hipEventCreateWithFlags(&s->build_done, hipEventDisableTiming));
+    if (s->timed) {
         for (auto &ev : s->kev) HIPCHK(hipEventCreate(&ev));
+        HIPCHK(hipEventCreate(&s->kev_prep_end));
+    }
     CHK(ensure_hap(e));
     if (s->Kmax == e->Nad && e->Ncorr == e->Nad) CHK(ensure_pmono(e));
     s->gen = e->panel_gen;
@@ -956,8 +1022,13 @@ int stream_setup(eimp_stream *s, size_t pool_bytes) {
     CHK(s->d_digest.ensure(nmax * 16, "digests"));
     CHK(s->d_blockpref.ensure(2 * (nmax + 1) * 4, "block prefix"));
     CHK(s->d_pitems.ensure(2 * nmax * sizeof(PbwtItem), "pbwt items"));
-    CHK(s->d_pool.ensure(pool_bytes, "output pool", false));
-    s->pool_cap = pool_bytes;
+    if (pool) {
+        s->pool = pool;
+    } else {
+        s->pool = std::make_shared<StreamPool>();
+        CHK(s->pool->buf.ensure(pool_bytes, "output pool", false));
+        s->pool->cap = pool_bytes;
+    }
     CHK(s->h_out.ensure(32 * nmax + nmax * sizeof(TgtDev)));
     // kernels and grids
     const uint32_t Kpad = (uint32_t)round_up(s->Kmax, 64);
@@ -995,18 +1066,46 @@ int stream_setup(eimp_stream *s, size_t pool_bytes) {
         T.stats = s->d_stats.as<uint32_t>() + 4 * t;
     }
     if (!(s->flags & (EIMP_STREAM_NO_GRAPH | EIMP_STREAM_TIMED))) {
-        HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-        const int rc = stream_launch_sequence(s);
-        hipGraph_t g = nullptr;
-        const hipError_t ec = hipStreamEndCapture(s->stream, &g);
-        if (rc != EIMP_OK) {
-            if (g) (void)hipGraphDestroy(g);
-            return rc;
-        }
-        if (ec != hipSuccess) return fail(EIMP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
-        s->graph = g;
-        HIPCHK(hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0));
+        CHK(stream_capture(s, stream_launch_prep, &s->graph_prep, &s->exec_prep));
+        CHK(stream_capture(s, stream_launch_build, &s->graph_build, &s->exec_build));
     }
+    return EIMP_OK;
+}
+
+int stream_create(eimp_engine *e, size_t max_targets, size_t max_window, size_t max_K,
+                  size_t pool_bytes, uint32_t flags, const std::shared_ptr<StreamPool> &pool,
+                  eimp_stream **out) {
+    if (!out) return fail(EIMP_ERR_INVALID, "null stream pointer");
+    *out = nullptr;
+    if (!e || !e->initialized) return fail(EIMP_ERR_STATE, "engine not initialized");
+    if (!e->have_cM) return fail(EIMP_ERR_STATE, "genetic positions not set (eimp_engine_set_panel_extras)");
+    if (max_targets < 1 || max_targets > 1024)
+        return fail(EIMP_ERR_INVALID, "max_targets must be in [1, 1024]");
+    if (max_window < 1 || max_window > e->M) return fail(EIMP_ERR_INVALID, "max_window must be in [1, M]");
+    if (max_K < 1) return fail(EIMP_ERR_INVALID, "max_K < 1");
+    if (!pool && pool_bytes < 256) return fail(EIMP_ERR_INVALID, "pool_bytes too small");
+    if (flags & ~(EIMP_BATCH_DIGEST | EIMP_BATCH_COMPACT | EIMP_BATCH_RAW | EIMP_STREAM_NO_GRAPH |
+                  EIMP_STREAM_TIMED))
+        return fail(EIMP_ERR_INVALID, "unknown stream flags");
+    if ((flags & EIMP_BATCH_COMPACT) && (flags & EIMP_BATCH_RAW))
+        return fail(EIMP_ERR_INVALID, "EIMP_BATCH_COMPACT and EIMP_BATCH_RAW exclude each other");
+    CHK(set_device(e));
+    eimp_stream *s = new eimp_stream();
+    s->e = e;
+    s->nmax = (uint32_t)max_targets;
+    s->Wmax = (uint32_t)max_window;
+    s->Kmax = (uint32_t)std::min<uint64_t>(max_K, e->Ncorr);
+    s->flags = flags;
+    s->timed = (flags & EIMP_STREAM_TIMED) != 0;
+    const int rc = stream_setup(s, pool_bytes, pool);
+    if (rc != EIMP_OK) {
+        const std::string msg = g_err;
+        stream_release(s);
+        delete s;
+        (void)hipGetLastError();
+        return fail(rc, msg);
+    }
+    *out = s;
     return EIMP_OK;
 }
 
@@ -1407,38 +1506,14 @@ int eimp_engine_mem_info(eimp_engine *e, size_t *free_bytes, size_t *total_bytes
 
 int eimp_stream_create(eimp_engine *e, size_t max_targets, size_t max_window, size_t max_K,
                        size_t pool_bytes, uint32_t flags, eimp_stream **out) {
-    if (!out) return fail(EIMP_ERR_INVALID, "null stream pointer");
-    *out = nullptr;
-    if (!e || !e->initialized) return fail(EIMP_ERR_STATE, "engine not initialized");
-    if (!e->have_cM) return fail(EIMP_ERR_STATE, "genetic positions not set (eimp_engine_set_panel_extras)");
-    if (max_targets < 1 || max_targets > 1024)
-        return fail(EIMP_ERR_INVALID, "max_targets must be in [1, 1024]");
-    if (max_window < 1 || max_window > e->M) return fail(EIMP_ERR_INVALID, "max_window must be in [1, M]");
-    if (max_K < 1) return fail(EIMP_ERR_INVALID, "max_K < 1");
-    if (pool_bytes < 256) return fail(EIMP_ERR_INVALID, "pool_bytes too small");
-    if (flags & ~(EIMP_BATCH_DIGEST | EIMP_BATCH_COMPACT | EIMP_BATCH_RAW | EIMP_STREAM_NO_GRAPH |
-                  EIMP_STREAM_TIMED))
-        return fail(EIMP_ERR_INVALID, "unknown stream flags");
-    if ((flags & EIMP_BATCH_COMPACT) && (flags & EIMP_BATCH_RAW))
-        return fail(EIMP_ERR_INVALID, "EIMP_BATCH_COMPACT and EIMP_BATCH_RAW exclude each other");
-    CHK(set_device(e));
-    eimp_stream *s = new eimp_stream();
-    s->e = e;
-    s->nmax = (uint32_t)max_targets;
-    s->Wmax = (uint32_t)max_window;
-    s->Kmax = (uint32_t)std::min<uint64_t>(max_K, e->Ncorr);
-    s->flags = flags;
-    s->timed = (flags & EIMP_STREAM_TIMED) != 0;
-    const int rc = stream_setup(s, pool_bytes);
-    if (rc != EIMP_OK) {
-        const std::string msg = g_err;
-        stream_release(s);
-        delete s;
-        (void)hipGetLastError();
-        return fail(rc, msg);
-    }
-    *out = s;
-    return EIMP_OK;
+    return stream_create(e, max_targets, max_window, max_K, pool_bytes, flags, nullptr, out);
+}
+
+int eimp_stream_create_shared(eimp_stream *other, size_t max_targets, size_t max_window,
+                              size_t max_K, uint32_t flags, eimp_stream **out) {
+    if (out) *out = nullptr;
+    if (!other) return fail(EIMP_ERR_INVALID, "null stream to share the pool of");
+    return stream_create(other->e, max_targets, max_window, max_K, 0, flags, other->pool, out);
 }
 
 int eimp_stream_submit(eimp_stream *s, size_t n, const eimp_target *targets, float cMmaxSplit) {
@@ -1490,8 +1565,15 @@ int eimp_stream_submit(eimp_stream *s, size_t n, const eimp_target *targets, flo
     if (gw) HIPCHK(hipMemcpyAsync(gtdev, gtst, gw * 8, hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipEventRecord(s->ev[1], s->stream));
     if (nt) {
-        if (s->exec) HIPCHK(hipGraphLaunch(s->exec, s->stream));
-        else CHK(stream_launch_sequence(s));
+        if (s->exec_prep) HIPCHK(hipGraphLaunch(s->exec_prep, s->stream));
+        else CHK(stream_launch_prep(s));
+        // the pool's previous build (this context's or another's) ends first
+        if (s->pool->last_build) HIPCHK(hipStreamWaitEvent(s->stream, s->pool->last_build, 0));
+        if (s->exec_build) HIPCHK(hipGraphLaunch(s->exec_build, s->stream));
+        else CHK(stream_launch_build(s));
+        HIPCHK(hipEventRecord(s->build_done, s->stream));
+        s->pool->last_build = s->build_done;
+        s->build_seq = ++s->pool->seq;
     }
     HIPCHK(hipEventRecord(s->ev[2], s->stream));
     if (nt) {
@@ -1557,8 +1639,11 @@ int eimp_stream_wait(eimp_stream *s, eimp_target_result *results, size_t *ndefer
     for (int c = 0; c < EIMP_NKCLASS; c++) s->kms[c] = 0;
     if (s->timed && nt) {
         for (int c = EIMP_KCLASS_DISTANCE; c < EIMP_KCLASS_COPY; c++) {
+            // the call locations end with the preparation: the wait for the
+            // pool's previous build, if any, belongs to no class
             float ms = 0;
-            HIPCHK(hipEventElapsedTime(&ms, s->kev[c], s->kev[c + 1]));
+            HIPCHK(hipEventElapsedTime(&ms, s->kev[c],
+                                       c == EIMP_KCLASS_SPLITS ? s->kev_prep_end : s->kev[c + 1]));
             s->kms[c] = ms;
         }
         float ms = 0;  // copies: header/table/genotypes in, results out
@@ -1583,6 +1668,8 @@ int eimp_stream_copy_target(eimp_stream *s, size_t t, uint32_t *besthaps, uint64
     if (!s) return fail(EIMP_ERR_INVALID, "null stream");
     if (s->pending || !s->collected) return fail(EIMP_ERR_STATE, "no collected submission");
     if (t >= s->n_last) return fail(EIMP_ERR_INVALID, "target index out of range");
+    if ((count0 || pbwt || cref_rows) && !s->outputs_current())
+        return fail(EIMP_ERR_STATE, kPoolReused);
     CHK(set_device(s->e));
     return copy_target_out(s->h_tg()[t], s->d_tg(), (uint32_t)t, s->e->Nad, s->stream, s->d_tmp,
                            besthaps, bhflags, splitsites, splits, cMdiffs, count0, pbwt, cref_rows);
@@ -1603,6 +1690,7 @@ int eimp_stream_digest_outputs(eimp_stream *s, uint64_t *digests) {
     if (s->pending || !s->collected) return fail(EIMP_ERR_STATE, "no collected submission");
     if (batch_layout(s->flags) == LAYOUT_RAW)
         return fail(EIMP_ERR_UNSUPPORTED, "raw rows carry no zero counts to digest");
+    if (!s->outputs_current()) return fail(EIMP_ERR_STATE, kPoolReused);
     CHK(set_device(s->e));
     const uint32_t nt = s->n_last;
     if (!nt) return EIMP_OK;

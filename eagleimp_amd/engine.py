@@ -31,6 +31,7 @@ EXPORTS = [
     "eimp_engine_run_batch", "eimp_engine_copy_target", "eimp_engine_set_timing",
     "eimp_engine_get_timing", "eimp_engine_reset_timing", "eimp_engine_set_pbwt_variant",
     "eimp_pbwt_variant_count", "eimp_pbwt_variant_info", "eimp_stream_create",
+    "eimp_stream_create_shared",
     "eimp_stream_submit", "eimp_stream_query", "eimp_stream_wait", "eimp_stream_copy_target",
     "eimp_stream_destroy", "eimp_engine_mem_info", "eimp_stream_kernel_ms",
     "eimp_engine_ref_pbwt", "eimp_phase_default_params", "eimp_phase_target", "eimp_phase_batch",
@@ -83,6 +84,7 @@ def lib():
         L.eimp_pbwt_variant_info.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                              C.POINTER(C.c_int)]
         L.eimp_stream_create.argtypes = [_P, _SZ, _SZ, _SZ, _SZ, C.c_uint32, C.POINTER(_P)]
+        L.eimp_stream_create_shared.argtypes = [_P, _SZ, _SZ, _SZ, C.c_uint32, C.POINTER(_P)]
         L.eimp_stream_submit.argtypes = [_P, _SZ, _P, C.c_float]
         L.eimp_stream_query.argtypes = [_P]
         L.eimp_stream_wait.argtypes = [_P, _P, C.POINTER(_SZ), _P]
@@ -297,9 +299,9 @@ class Engine:
         return f.value, t.value
 
     def stream(self, max_targets, max_window, max_K, pool_bytes, digest=True, compact=False,
-               graph=True, timed=False, raw=False) -> "Stream":
+               graph=True, timed=False, raw=False, share=None) -> "Stream":
         return Stream(self, max_targets, max_window, max_K, pool_bytes, digest, compact, graph,
-                      timed, raw)
+                      timed, raw, share)
 
     # -- measurement
     def set_timing(self, on=True):
@@ -365,7 +367,10 @@ class Stream:
     kernel classes, read with kernel_ms())."""
 
     def __init__(self, eng: Engine, max_targets, max_window, max_K, pool_bytes, digest=True,
-                 compact=False, graph=True, timed=False, raw=False):
+                 compact=False, graph=True, timed=False, raw=False, share=None):
+        """share: another Stream whose output pool this one uses (pool_bytes
+        is then ignored); their builds take turns, each preparation runs
+        beside the other's build."""
         self.eng = eng
         self._h = _P()
         self.layout = layout_name(compact, raw)
@@ -373,8 +378,12 @@ class Stream:
         flags = ((EIMP_BATCH_DIGEST if digest else 0) | (EIMP_BATCH_COMPACT if compact else 0)
                  | (EIMP_BATCH_RAW if raw else 0)
                  | (0 if graph else EIMP_STREAM_NO_GRAPH) | (EIMP_STREAM_TIMED if timed else 0))
-        _check(lib().eimp_stream_create(eng._h, max_targets, max_window, max_K, pool_bytes, flags,
-                                        C.byref(self._h)))
+        if share is not None:
+            _check(lib().eimp_stream_create_shared(share._h, max_targets, max_window, max_K,
+                                                   flags, C.byref(self._h)))
+        else:
+            _check(lib().eimp_stream_create(eng._h, max_targets, max_window, max_K, pool_bytes,
+                                            flags, C.byref(self._h)))
         self.max_targets = max_targets
         self.timed = timed
         self._n = 0

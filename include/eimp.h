@@ -219,6 +219,17 @@ typedef struct eimp_stream eimp_stream;
 int eimp_stream_create(eimp_engine *engine, size_t max_targets, size_t max_window, size_t max_K,
                        size_t pool_bytes, uint32_t flags, eimp_stream **stream);
 
+/* As eimp_stream_create, with the output pool of `other` (same engine)
+ * instead of a pool of its own.  The contexts sharing a pool take turns:
+ * each submission's build (planner, cref, PBWT) starts after the pool's
+ * previous build, whichever context submitted it, while its preparation
+ * (distances, top-K, monos, call locations) runs beside that build.  A
+ * context's outputs in the pool (count0, rows, cref) stay valid until any
+ * context sharing the pool submits again; copying them afterwards returns
+ * EIMP_ERR_STATE.  The pool lives until its last context is destroyed. */
+int eimp_stream_create_shared(eimp_stream *other, size_t max_targets, size_t max_window,
+                              size_t max_K, uint32_t flags, eimp_stream **stream);
+
 /* Queues a batch and returns without waiting.  On the device: the prep
  * kernels, a planner that sizes each target's outputs from its split count
  * and places them in the pool, then the cref and PBWT kernels pulling work
@@ -245,7 +256,8 @@ int eimp_stream_wait(eimp_stream *stream, eimp_target_result *results, size_t *n
 int eimp_stream_kernel_ms(eimp_stream *stream, double *ms);
 
 /* As eimp_engine_copy_target, for target t of the last collected submission
- * (outputs stay in the context's pool until its next submission). */
+ * (outputs stay in the context's pool until its next submission, or the next
+ * submission of a context sharing the pool). */
 int eimp_stream_copy_target(eimp_stream *stream, size_t t, uint32_t *besthaps, uint64_t *bhflags,
                             uint32_t *splitsites, uint64_t *splits, float *cMdiffs,
                             int32_t *count0, uint32_t *pbwt, uint64_t *cref_rows);

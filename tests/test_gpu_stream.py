@@ -236,3 +236,55 @@ def test_stream_argument_errors(built, small):
     st.wait()
     st.close()
     eng.close()
+
+
+@pytest.mark.parametrize("mode", ["graph", "timed"])
+def test_shared_pool_contexts_take_turns(built, small, mode):
+    """Two contexts on one output pool, submissions interleaved (the second
+    context's preparation beside the first one's build): every submission
+    equals the oracle; a context's pool outputs are refused once the other
+    context has submitted again; the pool outlives the context that made it."""
+    panel = small
+    pv = oracle.PanelView(panel.hap, panel.site, panel.cM)
+    eng = panel_engine(panel)
+    tg = synth.make_targets(panel, 53, 24, kset=[300, 64, 150, 299, 257], wmin=80, wmax=4000,
+                            miss=0.0, miss_hi=0.05)
+    kw = dict(timed=mode == "timed", graph=mode == "graph")
+    a = eng.stream(12, 4000, 300, 128 * MB, **kw)
+    b = eng.stream(12, 4000, 300, 0, share=a, **kw)
+    batches = [list(range(0, 10)), list(range(10, 20)), list(range(20, 24)), [5, 15, 0]]
+    ctx = [a, b, a, b]
+
+    def submit(st, idx):
+        st.submit([tg.gt[t] for t in idx], [tg.s0[t] for t in idx], [tg.W[t] for t in idx],
+                  [tg.K[t] for t in idx])
+
+    say("shared pool", mode, "submissions", len(batches))
+    submit(a, batches[0])
+    submit(b, batches[1])  # prepares while a builds
+    for i in range(len(batches)):
+        st, idx = ctx[i], batches[i]
+        res, nd, _ = st.wait()
+        assert nd == 0
+        refs = _refs(pv, tg, idx)
+        for j, t in enumerate(idx):
+            assert int(res.Mp[j]) == refs[j]["Mp"], f"submission {i} target {j}: Mp"
+            assert int(res.dig_pbwt[j]) == refs[j]["dig_pbwt"], f"submission {i} target {j}"
+            assert int(res.dig_count0[j]) == refs[j]["dig_count0"], f"submission {i} target {j}"
+        if i + 2 < len(batches):
+            # the other context has submitted since: these pool outputs are gone
+            with pytest.raises(EimpError):
+                st.copy_target(0)
+            with pytest.raises(EimpError):
+                st.digest_outputs()
+            submit(st, batches[i + 2])
+    # the last submission's outputs are current: rows equal the oracle's
+    _compare(ctx[-1], res, _refs(pv, tg, batches[-1]), batches[-1], "gpu", "shared pool, last")
+    # the pool outlives the context that allocated it
+    a.close()
+    submit(b, [2, 3])
+    res, nd, _ = b.wait()
+    refs = _refs(pv, tg, [2, 3])
+    assert [int(x) for x in res.dig_pbwt[:2]] == [r["dig_pbwt"] for r in refs]
+    b.close()
+    eng.close()
