@@ -708,7 +708,7 @@ __device__ __forceinline__ void cref_range_hap(const PanelDev &P, const TgtDev &
         const bool act = k < K;
         const uint32_t h = act ? (ident ? k : bh[k]) : 0u;
         const gptr_t<const uint32_t> row = gmem((const uint32_t *)P.hap) + (uint64_t)h * rowlen;
-        uint32_t cin = 0, acc = 0, j = 0, b = b0;
+        uint32_t cin = 0, acc = 0, j2 = 0, b = b0;  // j2: bit position of the next pair
         // row word x (x <= i1, inside the row) of window word x
         auto rword = [&](uint32_t x) -> uint32_t {
             const uint64_t q = q0 + x;
@@ -750,19 +750,22 @@ __device__ __forceinline__ void cref_range_hap(const PanelDev &P, const TgtDev &
                 cin = (uint32_t)(sum >> 32);
                 for (uint32_t sl = spw; sl; sl &= sl - 1) {
                     const uint32_t p = (uint32_t)__builtin_ctz(sl);
-                    acc |= ((C >> p) & 1u) << (2 * j);
-                    acc |= ((v >> p) & 1u) << (2 * j + 1);
-                    if (++j == 8) {
+                    // the split's two rows: inconsistency (bit 0), allele (bit 1)
+                    const uint32_t pair = __builtin_amdgcn_ubfe(C, p, 1u) |
+                                          (__builtin_amdgcn_ubfe(v, p, 1u) << 1);
+                    acc |= pair << j2;
+                    j2 += 2;
+                    if (j2 == 16) {
                         if (act) dst[(uint64_t)b * Kpad + k] = (uint16_t)acc;
                         acc = 0;
-                        j = 0;
+                        j2 = 0;
                         b++;
                     }
                 }
             }
         }
         if (last) {  // row 2S: an inconsistency after the last split
-            acc |= cin << (2 * j);
+            acc |= cin << j2;
             if (act) dst[(uint64_t)b * Kpad + k] = (uint16_t)acc;
         }
     }

@@ -900,7 +900,7 @@ private:
     const std::vector<fp> &cmd_;
     const std::vector<size_t> &calls_;
     const std::vector<Gt> &target_;
-    const std::vector<Constr> &cons_;  // read live: the driver fills it while this beam runs
+    const std::vector<Constr> &cons_;  // read live: the phasing loop fills it while this beam runs
     Cpbwt &pb_;
     bool rev_;
     int cursor_;
