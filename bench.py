@@ -347,10 +347,10 @@ def main():
                     help="stream contexts (device-driven, queue kernels) or the synchronous "
                          "batch engine")
     ap.add_argument("--contexts", type=int, default=2, help="stream contexts in flight")
-    ap.add_argument("--share-pool", action=argparse.BooleanOptionalAction, default=False,
+    ap.add_argument("--share-pool", choices=["auto", "on", "off"], default="auto",
                     help="when the device holds one context's pool: the contexts share it, "
-                         "each one's preparation beside the other's build (measured: +0.9%% "
-                         "at workload 4, +2.3%% at 3, -12%% at 2, so off by default)")
+                         "each one's preparation beside the other's build; auto times a few "
+                         "submissions both ways and keeps the faster")
     ap.add_argument("--layout", choices=["gpu", "compact"], default="gpu",
                     help="output rows held on the device: the reference GPU engine's drop-in "
                          "rows (4*Kwords words) or the CPU layout of src/PBWT.cpp "
@@ -493,18 +493,12 @@ def batch_bench(args) -> dict:
             # as many contexts as hold a full submission each
             per = 1.15 * batch * tuned["bytes_per_target"]
             nctx = int(max(1, min(args.contexts, budget // per)))
-        # one pool shared by the contexts when the device holds no more than
-        # one submission's pool (or no size is known): builds take turns and
+        # when the device holds no more than one submission's pool (or no size
+        # is known), the contexts may share that pool: builds take turns and
         # each preparation runs beside the other context's build
-        shared = args.share_pool and args.contexts > 1 and (tuned is None or nctx == 1)
-        if shared:
-            nctx = args.contexts
-        pool = int(budget // (1 if shared else nctx))
+        can_share = args.share_pool != "off" and args.contexts > 1 and (tuned is None or nctx == 1)
         maxW = int(tg.W.max())
         maxK = int(min(int(tg.K.max()), panel.N))
-        log(f"stream contexts: {nctx} x " + ("one shared " if shared else "") +
-            f"pool {pool / 2**30:.1f} GiB (device free {free / 2**30:.1f} of "
-            f"{total / 2**30:.1f} GiB)")
 
         def pipeline(sts, steps, first):
             """steps batches over the contexts, round robin; deferred targets are
@@ -548,14 +542,39 @@ def batch_bench(args) -> dict:
                 ci = (ci + 1) % len(sts)
             return out, kms, last
 
-        def contexts(timed):
+        def contexts(timed, shared=None, n=None):
+            shared = share if shared is None else shared
+            n = nctx if n is None else n
+            pool = int(budget // (1 if shared else n))
             out = []
-            for c in range(nctx):
+            for c in range(n):
                 out.append(eng.stream(batch, maxW, maxK, pool, graph=not timed, timed=timed,
                                       digest=False, compact=args.layout == "compact",
                                       share=out[0] if shared and c else None))
             return out
 
+        share = False
+        if can_share and args.share_pool == "on":
+            share = True
+        elif can_share:
+            # a few submissions each way (after one to warm up), the faster is timed
+            trial = {}
+            for way, n in ((False, nctx), (True, args.contexts)):
+                tsts = contexts(timed=True, shared=way, n=n)
+                pipeline(tsts, 1, 0)
+                t = time.perf_counter()
+                pipeline(tsts, 3, 1)
+                trial[way] = time.perf_counter() - t
+                for st in tsts:
+                    st.close()
+            share = trial[True] < trial[False]
+            log(f"shared pool trial: {trial[True]:.3f}s shared vs {trial[False]:.3f}s one pool "
+                f"per context -> {'shared' if share else 'one pool per context'}")
+        if share:
+            nctx = args.contexts
+        log(f"stream contexts: {nctx} x " + ("one shared " if share else "") +
+            f"pool {budget / (1 if share else nctx) / 2**30:.1f} GiB (device free "
+            f"{free / 2**30:.1f} of {total / 2**30:.1f} GiB)")
         sts = contexts(timed=True)
         log(f"warmup: {args.warmup} steps")
         pipeline(sts, args.warmup, 0)
@@ -737,7 +756,7 @@ def batch_bench(args) -> dict:
             "targets_per_step": batch, "K": sorted(set(int(k) for k in K_all)),
             "window": "full" if cfg.get("wmin") is None else "log-uniform",
             "mean_Mp": round(float(Mp_all.mean()), 1), "contexts": nctx,
-            "shared_pool": bool(args.engine == "stream" and shared),
+            "shared_pool": bool(args.engine == "stream" and share),
             "layout": args.layout if args.engine == "stream" else "gpu",
         },
         "bit_exact": bit_exact,

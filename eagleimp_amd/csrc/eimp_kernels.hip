@@ -269,10 +269,14 @@ __global__ __launch_bounds__(256) void k_distance(PanelDev P, const TgtDev *__re
 }
 
 // ---------------------------------------------------------------------------
-// a4: exact top-K selection / take-all.  grid n, block 1024
+// a4: exact top-K selection / take-all.  grid n, block SEL_NT.  Workgroups of
+// 256 threads fit on CUs beside a PBWT workgroup, so a context's selection can
+// run while another context's PBWT holds the device.
 // ---------------------------------------------------------------------------
-constexpr int SEL_NT = 1024;
+constexpr int SEL_NT = 256;
 constexpr int SEL_NW = SEL_NT / 64;
+constexpr int SEL_BINS = 1024;                    // 10-bit digits
+constexpr int SEL_BPT = SEL_BINS / SEL_NT;        // bins per thread in the bin scan
 
 // block-wide exclusive scan of one value per thread; also returns the total
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *s_w, uint32_t &total) {
@@ -296,7 +300,7 @@ __global__ __launch_bounds__(SEL_NT) void k_select(PanelDev P, const TgtDev *__r
                                                    const uint32_t *__restrict__ d_n) {
     if (d_n && blockIdx.x >= *d_n) return;
     const TgtDev &T = tg[blockIdx.x];
-    __shared__ uint32_t hist[1024];
+    __shared__ uint32_t hist[SEL_BINS];
     __shared__ uint32_t s_w[SEL_NW];
     __shared__ uint32_t s_bin, s_before;
     const uint32_t tid = threadIdx.x, wave = tid >> 6, l = lane_id();
@@ -330,11 +334,23 @@ __global__ __launch_bounds__(SEL_NT) void k_select(PanelDev P, const TgtDev *__r
                     atomicAdd(&hist[(v >> lo) & (nb - 1)], 1u);
             }
             __syncthreads();
-            uint32_t c = tid < nb ? hist[tid] : 0u, total;
+            // thread tid scans bins SEL_BPT*tid .. SEL_BPT*tid + SEL_BPT-1
+            uint32_t cb[SEL_BPT], c = 0, total;
+#pragma unroll
+            for (int q = 0; q < SEL_BPT; q++) {
+                const uint32_t bin = SEL_BPT * tid + (uint32_t)q;
+                cb[q] = bin < nb ? hist[bin] : 0u;
+                c += cb[q];
+            }
             uint32_t ex = block_excl_scan(c, s_w, total);
-            if (tid < nb && ex < k && k <= ex + c) {
-                s_bin = tid;
-                s_before = ex;
+#pragma unroll
+            for (int q = 0; q < SEL_BPT; q++) {
+                const uint32_t bin = SEL_BPT * tid + (uint32_t)q;
+                if (bin < nb && ex < k && k <= ex + cb[q]) {
+                    s_bin = bin;
+                    s_before = ex;
+                }
+                ex += cb[q];
             }
             __syncthreads();
             k -= s_before;
@@ -381,16 +397,20 @@ __global__ __launch_bounds__(SEL_NT) void k_select(PanelDev P, const TgtDev *__r
 
 // ---------------------------------------------------------------------------
 // a5: monos for targets that select a subset.  grid (ceil(Ww/64), n), block
-// 1024: 16 waves split the K chosen rows, lanes cover 64 window words.
+// MONO_NT: the waves split the K chosen rows, lanes cover 64 window words
+// (256 threads: they fit beside a PBWT workgroup, as the selection does).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_monos(PanelDev P, const TgtDev *__restrict__ tg,
-                                               const uint32_t *__restrict__ tlist,
-                                               const uint32_t *__restrict__ d_n) {
+constexpr int MONO_NT = 256;
+constexpr int MONO_NW = MONO_NT / 64;
+
+__global__ __launch_bounds__(MONO_NT) void k_monos(PanelDev P, const TgtDev *__restrict__ tg,
+                                                   const uint32_t *__restrict__ tlist,
+                                                   const uint32_t *__restrict__ d_n) {
     // tlist: targets that need monos; without it every target of the batch (size *d_n)
     if (!tlist && blockIdx.y >= *d_n) return;
     const TgtDev &T = tg[tlist ? tlist[blockIdx.y] : blockIdx.y];
     if (T.flags & TF_IDENTITY) return;
-    __shared__ uint64_t s_and[16][64], s_or[16][64];
+    __shared__ uint64_t s_and[MONO_NW][64], s_or[MONO_NW][64];
     const uint32_t wave = threadIdx.x >> 6, l = lane_id();
     const uint64_t Ww = (T.W + 63) / 64;
     const uint64_t i = (uint64_t)blockIdx.x * 64 + l;
@@ -399,15 +419,18 @@ __global__ __launch_bounds__(1024) void k_monos(PanelDev P, const TgtDev *__rest
     const uint64_t bitoff = T.s0 + 64 * i;
     if (valid) {
         uint32_t k = wave;
-        for (; k + 48 < T.K; k += 64) {
+        for (; k + 3 * MONO_NW < T.K; k += 4 * MONO_NW) {
             uint64_t v0 = funnel(P.hap + (uint64_t)T.bh[k] * P.hap_stride, bitoff, P.hap_stride);
-            uint64_t v1 = funnel(P.hap + (uint64_t)T.bh[k + 16] * P.hap_stride, bitoff, P.hap_stride);
-            uint64_t v2 = funnel(P.hap + (uint64_t)T.bh[k + 32] * P.hap_stride, bitoff, P.hap_stride);
-            uint64_t v3 = funnel(P.hap + (uint64_t)T.bh[k + 48] * P.hap_stride, bitoff, P.hap_stride);
+            uint64_t v1 = funnel(P.hap + (uint64_t)T.bh[k + MONO_NW] * P.hap_stride, bitoff,
+                                 P.hap_stride);
+            uint64_t v2 = funnel(P.hap + (uint64_t)T.bh[k + 2 * MONO_NW] * P.hap_stride, bitoff,
+                                 P.hap_stride);
+            uint64_t v3 = funnel(P.hap + (uint64_t)T.bh[k + 3 * MONO_NW] * P.hap_stride, bitoff,
+                                 P.hap_stride);
             a &= v0 & v1 & v2 & v3;
             o |= v0 | v1 | v2 | v3;
         }
-        for (; k < T.K; k += 16) {
+        for (; k < T.K; k += MONO_NW) {
             uint64_t v = funnel(P.hap + (uint64_t)T.bh[k] * P.hap_stride, bitoff, P.hap_stride);
             a &= v;
             o |= v;
@@ -417,7 +440,7 @@ __global__ __launch_bounds__(1024) void k_monos(PanelDev P, const TgtDev *__rest
     s_or[wave][l] = o;
     __syncthreads();
     if (wave == 0 && valid) {
-        for (int w = 1; w < 16; w++) {
+        for (int w = 1; w < MONO_NW; w++) {
             a &= s_and[w][l];
             o |= s_or[w][l];
         }
@@ -548,7 +571,7 @@ __global__ __launch_bounds__(64) void k_splits_clear(const TgtDev *__restrict__ 
 
 // ---------------------------------------------------------------------------
 // bitmap -> ascending index list (drop-in path: best-haps flags, split
-// bitmap).  grid n, block 1024.  which: 0 best haps, 1 split sites.
+// bitmap).  grid n, block SEL_NT.  which: 0 best haps, 1 split sites.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(SEL_NT) void k_bits_to_list(const TgtDev *__restrict__ tg,
                                                         uint32_t nbits_bh, int which) {
@@ -1761,7 +1784,7 @@ hipError_t launch_select(const PanelDev &P, const TgtDev *tg, uint32_t n, hipStr
 hipError_t launch_monos(const PanelDev &P, const TgtDev *tg, const uint32_t *tlist, uint32_t n,
                         uint32_t max_ww, hipStream_t s) {
     if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_monos, dim3((max_ww + 63) / 64, n), dim3(1024), 0, s, P, tg, tlist,
+    hipLaunchKernelGGL(k_monos, dim3((max_ww + 63) / 64, n), dim3(MONO_NT), 0, s, P, tg, tlist,
                        (const uint32_t *)nullptr);
     return hipGetLastError();
 }
@@ -1910,7 +1933,7 @@ hipError_t launch_prep_dev(int cls, const PanelDev &P, const TgtDev *tg, const u
         hipLaunchKernelGGL(k_select, dim3(nmax), dim3(SEL_NT), 0, s, P, tg, d_n);
         break;
     case 2:
-        hipLaunchKernelGGL(k_monos, dim3((max_ww + 63) / 64, nmax), dim3(1024), 0, s, P, tg,
+        hipLaunchKernelGGL(k_monos, dim3((max_ww + 63) / 64, nmax), dim3(MONO_NT), 0, s, P, tg,
                            (const uint32_t *)nullptr, d_n);
         break;
     default:
