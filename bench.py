@@ -677,6 +677,26 @@ def batch_bench(args) -> dict:
                        and np.array_equal(got["count0"], r1["count0"]))
             rows_compared = 1
             log(f"stored rows of target {calib} vs the oracle's, word for word: {rows_ok}")
+        pbwt_ref = None
+        if full and oracle.ref_available():
+            # the PBWT half of the CPU path with the reference's own PBWT.cpp
+            # (built from its sources) beside the port, on this target's
+            # condensed reference, one core each
+            kk = int(tg.K[calib])
+            t = time.perf_counter()
+            ref_rows, _ = oracle.ref_build_pbwt(r1["cref"], kk, r1["count0"])
+            t_ref = time.perf_counter() - t
+            t = time.perf_counter()
+            port_rows = oracle.build_pbwt(r1["cref"], kk, r1["count0"])
+            t_port = time.perf_counter() - t
+            pbwt_ref = {"reference_s": round(t_ref, 3), "port_s": round(t_port, 3),
+                        "rows": int(r1["Mp"]), "equal": bool(np.array_equal(ref_rows, port_rows)),
+                        "what": "buildComprPBWT of one target's condensed reference (forward + "
+                                "backward), src/PBWT.cpp compiled in place vs the oracle's "
+                                "restatement, one core"}
+            del ref_rows, port_rows
+            log(f"cpu PBWT half, one target: reference {t_ref:.2f}s, port {t_port:.2f}s, "
+                f"equal {pbwt_ref['equal']}")
         r1 = {"Mp": [r1["Mp"]], "dig_pbwt": [r1["dig_pbwt"]], "dig_count0": [r1["dig_count0"]]}
         n = int(max(1, cores * max(1, round(args.cpu_seconds / max(one, 1e-3)))))
         ids = pool_ids[1:1 + n] if len(pool_ids) > 1 else pool_ids
@@ -690,6 +710,8 @@ def batch_bench(args) -> dict:
                "sample": f"{n} targets of the same workload (oracle C restatement of "
                          f"src/Target.cpp:779-1059 + src/PBWT.cpp:207-285, OpenMP dynamic,1 over "
                          f"targets, {dt:.1f}s); 1 target on 1 core took {one:.2f}s"}
+        if pbwt_ref is not None:
+            cpu["pbwt_reference"] = pbwt_ref
         log(f"cpu baseline: {cpu}")
         if post:
             bit_exact = True
