@@ -908,6 +908,9 @@ __global__ __launch_bounds__(CREF_NT) void k_cref_ident(PanelDev P, const TgtDev
 //           cumulative-zero pairs
 //   (barrier)
 //   pass 3  read back the payloads of my positions.
+// A row whose K bits are all equal keeps the order: pass 2, the second
+// barrier and pass 3 are skipped (the wave counts alternate between two LDS
+// halves, so the next step's counts never overwrite ones still being read).
 // At each 16-row block boundary the next block's uint16 values are staged in
 // LDS and gathered by hap index.
 // ---------------------------------------------------------------------------
@@ -971,7 +974,9 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
     // Kpad), staged at block boundaries (when no payloads are in flight) in
     // the first half of pay[]
     uint16_t *stage = (uint16_t *)smem;
-    uint32_t *wzero = smem + NPOS;                 // NW: zeros of each wave's chunks
+    // 2 x NW: zeros of each wave's chunks, by step parity (a step that moves
+    // nothing has no second barrier, so the next step writes the other half)
+    uint32_t *wzero2 = smem + NPOS;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = lane_id();
     const uint32_t ch0 = wave * NS;                // first chunk of this wave
     const gptr_t<const uint16_t> cref = gmem((const uint16_t *)T.cref);
@@ -1075,6 +1080,7 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
         // every lane (one address, one value)
         const uint32_t zl = 64u - (uint32_t)__popc(olo) - (uint32_t)__popc(ohi);
         const uint32_t zex = lane_scan_excl<NS>(zl);
+        uint32_t *wzero = wzero2 + (step & 1u) * (uint32_t)NW;
         wzero[wave] = (uint32_t)__builtin_amdgcn_readlane((int)(zex + zl), NS - 1);
         __syncthreads();
         // zeros before my first chunk, and in the whole row (count0)
@@ -1099,33 +1105,39 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
         c0acc = eimp_llvm_writelane(cnt0, step & 15u, c0acc);
         // lane c: zeros before chunk ch0 + c (lanes >= NS: don't care)
         const uint32_t zbv = run + zex;
-        // pass 2: stable partition, zeros first.  run / ob: positions of the
-        // zeros / ones of the row before the current chunk (ones go after all
-        // cnt0 zeros).  My bit is bit ln of the ballot (inverse ballot: the
-        // mask itself selects); ones go to ob + (ones below me), zeros to
-        // run + (zeros below me) = run + |ln - t|.  No branch on the chunk's
-        // contents: the step is bound by the instructions its waves issue,
-        // and a test for all-equal chunks costs more (scalar compare and
-        // taken branches) than the vector instructions it would skip.
-        {
-            uint32_t ob = cnt0 + ch0 * 64u - run;
-            static_for<0, NS>([&](auto ci) {
-                constexpr int c = decltype(ci)::value;
-                const uint64_t m = mk[c];
-                const uint32_t ones = (uint32_t)__popcll(m);
-                const uint32_t t = mbcnt(m);  // ones below me
-                const uint32_t od = ob + t;
-                const uint32_t zd = __builtin_amdgcn_sad_u16(ln, t, run);
-                // LDS byte address of my destination in pay[]
-                const uint32_t addr = (__builtin_amdgcn_inverse_ballot_w64(m) ? od : zd) * 4u;
-                *(lds_ptr_t<uint32_t>)(size_t)addr = p[c];
-                run += 64u - ones;
-                ob += ones;
-            });
-        }
-        __syncthreads();
+        // A row whose K bits are all equal (about a fifth of the rows on
+        // these panels) leaves the order as it is: no partition, no second
+        // barrier, no read-back.  cnt0 is the same in every wave, so the whole
+        // workgroup takes the same branch.
+        if (cnt0 != 0u && cnt0 != K) {
+            // pass 2: stable partition, zeros first.  run / ob: positions of the
+            // zeros / ones of the row before the current chunk (ones go after all
+            // cnt0 zeros).  My bit is bit ln of the ballot (inverse ballot: the
+            // mask itself selects); ones go to ob + (ones below me), zeros to
+            // run + (zeros below me) = run + |ln - t|.  No branch on the chunk's
+            // contents: the step is bound by the instructions its waves issue,
+            // and a test for all-equal chunks costs more (scalar compare and
+            // taken branches) than the vector instructions it would skip.
+            {
+                uint32_t ob = cnt0 + ch0 * 64u - run;
+                static_for<0, NS>([&](auto ci) {
+                    constexpr int c = decltype(ci)::value;
+                    const uint64_t m = mk[c];
+                    const uint32_t ones = (uint32_t)__popcll(m);
+                    const uint32_t t = mbcnt(m);  // ones below me
+                    const uint32_t od = ob + t;
+                    const uint32_t zd = __builtin_amdgcn_sad_u16(ln, t, run);
+                    // LDS byte address of my destination in pay[]
+                    const uint32_t addr = (__builtin_amdgcn_inverse_ballot_w64(m) ? od : zd) * 4u;
+                    *(lds_ptr_t<uint32_t>)(size_t)addr = p[c];
+                    run += 64u - ones;
+                    ob += ones;
+                });
+            }
+            __syncthreads();
 #pragma unroll
-        for (int c = 0; c < NS; c++) p[c] = pay[(ch0 + (uint32_t)c) * 64 + ln];
+            for (int c = 0; c < NS; c++) p[c] = pay[(ch0 + (uint32_t)c) * 64 + ln];
+        }
         if constexpr (ABS) {
             if (absperm) {
                 const gptr_t<int32_t> ap = gmem(absperm) + (uint64_t)(step + 1) * K;
@@ -1202,7 +1214,7 @@ __device__ __forceinline__ void pbwt_target(const TgtDev &T, const bool rev, uin
 // register budget is capped so that VGPRs do not lower the occupancy (512 x 20:
 // three workgroups per CU need at most 85 VGPRs).
 constexpr int pbwt_waves_hint(int NT, int CPW) {
-    const int lds = CPW * NT * 4 + (NT / 64) * 4;
+    const int lds = CPW * NT * 4 + 2 * (NT / 64) * 4;
     const int w = (163840 / lds) * (NT / 256);
     return w < 1 ? 1 : (w > 6 ? 6 : w);
 }
@@ -1881,7 +1893,7 @@ size_t pbwt_lds_bytes(int v, uint32_t Kpad) {
     const int ns = pbwt_slots_for(Kpad, nt);
     if (ns < 0 || ns > kPbwtVariants[v].cpw) return ~(size_t)0;
     const size_t npos = (size_t)ns * nt;  // NS x NW x 64
-    return npos * 4 + (nt / 64) * 4;      // payloads (the cref stage shares them), wave counts
+    return npos * 4 + 2 * (nt / 64) * 4;  // payloads (the cref stage shares them), wave counts
 }
 hipError_t pbwt_prepare(int v, size_t lds) {
     hipError_t e = pbwt_lds_at_zero((const void *)kPbwtVariants[v].fn);
@@ -1912,7 +1924,7 @@ hipError_t launch_panel_pbwt(const PanelDev &P, uint64_t m0, uint32_t nsites, ui
     const uint32_t nw = (P.N_ad + 63) / 64;
     hipLaunchKernelGGL(k_site_blocks, dim3((nsites + 15) / 16, (nw + 255) / 256), dim3(256), 0, s, P,
                        m0, nsites, blocks, T.Kpad);
-    const size_t lds = (size_t)ns * NT * 4 + (NT / 64) * 4;
+    const size_t lds = (size_t)ns * NT * 4 + 2 * (NT / 64) * 4;
     hipError_t e = pbwt_lds_at_zero((const void *)k_panel_pbwt<NT, CPW>);
     if (e != hipSuccess) return e;
     e = raise_lds_limit((const void *)k_panel_pbwt<NT, CPW>, lds);
