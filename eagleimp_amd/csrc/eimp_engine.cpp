@@ -850,9 +850,14 @@ int run_batch_impl(eimp_engine *e, size_t n, const eimp_target *targets, float c
 // build.  A context's pool outputs stay valid until a context sharing the
 // pool submits again.
 // ---------------------------------------------------------------------------
+// Contexts sharing a pool may be driven from different host threads: `mu`
+// guards seq and last_build, and is held from the wait on the previous build
+// to the record of the new one, and across every synchronous read of pool
+// rows, so a build is never enqueued while another context reads them.
 struct StreamPool {
     DevBuf buf;
     uint64_t cap = 0;
+    std::mutex mu;
     uint64_t seq = 0;                 // builds submitted into the pool
     hipEvent_t last_build = nullptr;  // the last submitted build's event (its context owns it)
 };
@@ -885,7 +890,7 @@ struct eimp_stream {
     uint32_t n_last = 0;
     uint64_t build_seq = 0;  // pool->seq of this context's last build
 
-    // the pool still holds this context's last outputs
+    // the pool still holds this context's last outputs (pool->mu held)
     bool outputs_current() const { return pool && build_seq == pool->seq; }
 
     BatchDev *d_batch() const { return d_in.as<BatchDev>(); }
@@ -947,8 +952,11 @@ int stream_launch_build(eimp_stream *s) {
 void stream_release(eimp_stream *s) {
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     if (s->pool) {
-        // the build event dies with this context: later builds need not wait for it
-        if (s->build_done && s->pool->last_build == s->build_done) s->pool->last_build = nullptr;
+        {
+            // the build event dies with this context: later builds need not wait for it
+            std::lock_guard<std::mutex> lk(s->pool->mu);
+            if (s->build_done && s->pool->last_build == s->build_done) s->pool->last_build = nullptr;
+        }
         if (s->pool.use_count() == 1) s->pool->buf.release();
         s->pool.reset();
     }
@@ -1568,6 +1576,7 @@ int eimp_stream_submit(eimp_stream *s, size_t n, const eimp_target *targets, flo
         if (s->exec_prep) HIPCHK(hipGraphLaunch(s->exec_prep, s->stream));
         else CHK(stream_launch_prep(s));
         // the pool's previous build (this context's or another's) ends first
+        std::lock_guard<std::mutex> lk(s->pool->mu);
         if (s->pool->last_build) HIPCHK(hipStreamWaitEvent(s->stream, s->pool->last_build, 0));
         if (s->exec_build) HIPCHK(hipGraphLaunch(s->exec_build, s->stream));
         else CHK(stream_launch_build(s));
@@ -1668,8 +1677,12 @@ int eimp_stream_copy_target(eimp_stream *s, size_t t, uint32_t *besthaps, uint64
     if (!s) return fail(EIMP_ERR_INVALID, "null stream");
     if (s->pending || !s->collected) return fail(EIMP_ERR_STATE, "no collected submission");
     if (t >= s->n_last) return fail(EIMP_ERR_INVALID, "target index out of range");
-    if ((count0 || pbwt || cref_rows) && !s->outputs_current())
-        return fail(EIMP_ERR_STATE, kPoolReused);
+    // the copies are synchronous: holding the lock keeps other builds out of the pool meanwhile
+    std::unique_lock<std::mutex> lk;
+    if (count0 || pbwt || cref_rows) {
+        lk = std::unique_lock<std::mutex>(s->pool->mu);
+        if (!s->outputs_current()) return fail(EIMP_ERR_STATE, kPoolReused);
+    }
     CHK(set_device(s->e));
     return copy_target_out(s->h_tg()[t], s->d_tg(), (uint32_t)t, s->e->Nad, s->stream, s->d_tmp,
                            besthaps, bhflags, splitsites, splits, cMdiffs, count0, pbwt, cref_rows);
@@ -1690,6 +1703,7 @@ int eimp_stream_digest_outputs(eimp_stream *s, uint64_t *digests) {
     if (s->pending || !s->collected) return fail(EIMP_ERR_STATE, "no collected submission");
     if (batch_layout(s->flags) == LAYOUT_RAW)
         return fail(EIMP_ERR_UNSUPPORTED, "raw rows carry no zero counts to digest");
+    std::lock_guard<std::mutex> lk(s->pool->mu);  // held until the digests are read back
     if (!s->outputs_current()) return fail(EIMP_ERR_STATE, kPoolReused);
     CHK(set_device(s->e));
     const uint32_t nt = s->n_last;

@@ -288,3 +288,60 @@ def test_shared_pool_contexts_take_turns(built, small, mode):
     assert [int(x) for x in res.dig_pbwt[:2]] == [r["dig_pbwt"] for r in refs]
     b.close()
     eng.close()
+
+
+def test_shared_pool_contexts_on_two_threads(built, small):
+    """Two contexts on one output pool, each driven by its own host thread
+    (the library calls release the interpreter lock, so the submissions and
+    reads interleave freely): every submission's digests equal the oracle's,
+    and a read of pool rows either returns the oracle's rows or is refused
+    because the other context has built since -- never rows half overwritten."""
+    import threading
+    panel = small
+    pv = oracle.PanelView(panel.hap, panel.site, panel.cM)
+    eng = panel_engine(panel)
+    tg = synth.make_targets(panel, 59, 24, kset=[300, 64, 150, 299, 257], wmin=80, wmax=4000,
+                            miss=0.0, miss_hi=0.05)
+    refs = _refs(pv, tg, range(24))
+    a = eng.stream(6, 4000, 300, 64 * MB)
+    b = eng.stream(6, 4000, 300, 0, share=a)
+    plans = {0: [list(range(0, 6)), list(range(12, 18)), [1, 3, 5]] * 2,
+             1: [list(range(6, 12)), list(range(18, 24)), [20, 2]] * 2}
+    tally = {0: [0, 0], 1: [0, 0]}  # (row reads that matched, reads refused)
+    errors = []
+
+    def drive(k, st):
+        try:
+            for i, idx in enumerate(plans[k]):
+                st.submit([tg.gt[t] for t in idx], [tg.s0[t] for t in idx],
+                          [tg.W[t] for t in idx], [tg.K[t] for t in idx])
+                res, nd, _ = st.wait()
+                assert nd == 0
+                for j, t in enumerate(idx):
+                    assert int(res.dig_pbwt[j]) == refs[t]["dig_pbwt"], f"thread {k} sub {i} tgt {j}"
+                    assert int(res.dig_count0[j]) == refs[t]["dig_count0"], f"thread {k} sub {i}"
+                j = i % len(idx)
+                try:
+                    got = st.copy_target(j, cref=True)
+                except EimpError:
+                    tally[k][1] += 1
+                    continue
+                check_target(refs[idx[j]], got, int(refs[idx[j]]["K"]), "gpu", f"thread {k} sub {i}")
+                tally[k][0] += 1
+        except BaseException as exc:  # reported by the main thread
+            errors.append(exc)
+
+    say("shared pool on two threads, submissions", len(plans[0]) + len(plans[1]))
+    th = [threading.Thread(target=drive, args=(k, st)) for k, st in ((0, a), (1, b))]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join(timeout=100)
+    assert not any(x.is_alive() for x in th), "a driving thread did not finish"
+    if errors:
+        raise errors[0]
+    say("row reads matched / refused", tally)
+    assert sum(tally[0]) == len(plans[0]) and sum(tally[1]) == len(plans[1])
+    a.close()
+    b.close()
+    eng.close()
