@@ -943,6 +943,10 @@ __device__ __forceinline__ uint32_t lane_scan_excl(uint32_t v) {
     return x - v;
 }
 
+// LDS bytes before the PBWT payloads: 32 (NS - 1) words, the largest amount a
+// slot's store address sits below its destination (pbwt_steps, pass 2).
+__host__ __device__ constexpr uint32_t pbwt_pay_bias(int ns) { return 128u * (uint32_t)(ns - 1); }
+
 // NS = slots per wave for this target.  Wave w owns chunks [w*NS, w*NS+NS);
 // chunks >= Kpad/64 are padding (all-ones bits, never moving), so every slot
 // is processed unconditionally.
@@ -956,8 +960,9 @@ __device__ __forceinline__ uint32_t lane_scan_excl(uint32_t v) {
 // autotuner picks between the variants built with each.
 //
 // The kernels that call this use dynamic LDS only (the host checks that they
-// have no static LDS), so pay[] starts at LDS address 0 and the scattered
-// stores address it with plain byte offsets.
+// have no static LDS), so LDS byte addresses are plain offsets from 0: pay[]
+// starts at byte pbwt_pay_bias(NS), which keeps pass 2's store addresses
+// (before their immediate offsets) non-negative.
 template <int NT, int NS, bool ABS = false, int WS = 0>
 __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint32_t *smem,
                                            int32_t *__restrict__ absperm = nullptr,
@@ -969,16 +974,21 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
     const uint32_t nch = Kpad / 64;
     const uint32_t npairs = (K + 31) / 32;
     const uint32_t KW = 2 * npairs;
-    uint32_t *pay = smem;                          // NPOS payloads, LDS bytes [0, 4*NPOS)
+    // NPOS payloads at LDS bytes [PBIAS, PBIAS + 4*NPOS): the partition's
+    // stores address them as 4 sel + 128 c (pass 2), sel >= 0
+    constexpr uint32_t PBIAS = pbwt_pay_bias(NS);
+    uint32_t *pay = smem + PBIAS / 4;
     // Kpad + 2 uint16: the next cref block by haplotype (and all ones at id
-    // Kpad), staged at block boundaries (when no payloads are in flight) in
-    // the first half of pay[]
+    // Kpad), staged at block boundaries (when no payloads are in flight) from
+    // LDS byte 0
     uint16_t *stage = (uint16_t *)smem;
     // 2 x NW: zeros of each wave's chunks, by step parity (a step that moves
     // nothing has no second barrier, so the next step writes the other half)
-    uint32_t *wzero2 = smem + NPOS;
+    uint32_t *wzero2 = pay + NPOS;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = lane_id();
     const uint32_t ch0 = wave * NS;                // first chunk of this wave
+    // pass 2's per-lane constant (bytes): 4 (64 ch0 + ln + 64 (NS - 1))
+    const uint32_t vbase4 = 4u * (ch0 * 64u + ln + 64u * (NS - 1));
     const gptr_t<const uint16_t> cref = gmem((const uint16_t *)T.cref);
     // wave-uniform flags as scalars: their branches stay scalar branches
     const bool dig = __builtin_amdgcn_readfirstlane(T.digest != nullptr ? 1u : 0u) != 0;
@@ -1044,19 +1054,50 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
     uint32_t nextb = rev ? ((Mp - 1) & 15) + 1 : 16u;
     // forward pass: count0 of the current 16-row block, lane (step & 15)
     uint32_t c0acc = 0;
+    // PF: the next block boundary's cref block (Kpad uint16 = Kpad/2 words,
+    // at most NPF per thread since Kpad <= NS * NT) is loaded into registers
+    // one block ahead, so staging it waits for no global load.  Only the
+    // 1024-thread shapes (one workgroup per CU, 128 VGPRs a wave) have the
+    // registers; the others load at the boundary.
+    constexpr bool PF = NT >= 1024;
+    constexpr int NPF = PF ? (NS + 1) / 2 : 1;
+    uint32_t pf[NPF];
+    auto block_src = [&](uint32_t st) {
+        const uint32_t r = rev ? Mp - 1 - st : st;
+        return (gptr_t<const uint32_t>)(cref + (uint64_t)(r >> 4) * Kpad);
+    };
+    auto prefetch = [&](uint32_t st) {
+        const gptr_t<const uint32_t> src = block_src(st);
+#pragma unroll
+        for (int i = 0; i < NPF; i++) {
+            const uint32_t q = threadIdx.x + (uint32_t)i * NT;
+            pf[i] = q < Kpad / 2 ? src[q] : 0u;
+        }
+    };
+    if constexpr (PF) {
+        if (nextb < Mp) prefetch(nextb);
+    }
     for (uint32_t step = 0; step < Mp; ++step) {
         if (step == nextb) {
             nextb += 16;
-            const uint32_t r = rev ? Mp - 1 - step : step;
             jbit = rev ? 0x8000u : 1u;
             if (!rev && wave == 0 && ln < 16u) {  // the previous block's count0
                 if constexpr (ABS) gmem(c0out)[step - 16 + ln] = (int32_t)c0acc;
                 else gmem(T.count0)[step - 16 + ln] = (int32_t)c0acc;
             }
-            const gptr_t<const uint32_t> src = (gptr_t<const uint32_t>)(cref + (uint64_t)(r >> 4) * Kpad);
             uint32_t *st32 = (uint32_t *)stage;
             __syncthreads();  // every wave has read back its payloads
-            for (uint32_t q = threadIdx.x; q < Kpad / 2; q += NT) st32[q] = src[q];
+            if constexpr (PF) {
+#pragma unroll
+                for (int i = 0; i < NPF; i++) {
+                    const uint32_t q = threadIdx.x + (uint32_t)i * NT;
+                    if (q < Kpad / 2) st32[q] = pf[i];
+                }
+                if (nextb < Mp) prefetch(nextb);
+            } else {
+                const gptr_t<const uint32_t> src = block_src(step);
+                for (uint32_t q = threadIdx.x; q < Kpad / 2; q += NT) st32[q] = src[q];
+            }
             if (threadIdx.x == 0) st32[Kpad / 2] = ~0u;  // id Kpad: the padding chunks
             __syncthreads();
 #pragma unroll
@@ -1110,28 +1151,36 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
         // barrier, no read-back.  cnt0 is the same in every wave, so the whole
         // workgroup takes the same branch.
         if (cnt0 != 0u && cnt0 != K) {
-            // pass 2: stable partition, zeros first.  run / ob: positions of the
-            // zeros / ones of the row before the current chunk (ones go after all
-            // cnt0 zeros).  My bit is bit ln of the ballot (inverse ballot: the
-            // mask itself selects); ones go to ob + (ones below me), zeros to
-            // run + (zeros below me) = run + |ln - t|.  No branch on the chunk's
-            // contents: the step is bound by the instructions its waves issue,
-            // and a test for all-equal chunks costs more (scalar compare and
-            // taken branches) than the vector instructions it would skip.
+            // pass 2: stable partition, zeros first.  For slot c let ob_c / run_c
+            // be the destinations of the row's first one / zero at or after the
+            // chunk (ob_c = cnt0 + ones before it, run_c = zeros before it, so
+            // ob_c + run_c = cnt0 + 64 (ch0 + c)) and t the ones below me in the
+            // chunk: a one goes to ob_c + t, a zero to run_c + ln - t.  With
+            //   x = ob_c + t + 32 (NS - 1 - c)
+            // dest + 32 (NS - 1 - c) is x for a one and V - x for a zero, with
+            // V = cnt0 + 64 ch0 + ln + 64 (NS - 1) the same for every slot, so
+            //   PBIAS + 4 dest = 4 sel + 128 c,  sel = bit ? x : V - x >= 0,
+            // and 128 c is the store's immediate offset (pay[] starts at byte
+            // PBIAS = 128 (NS - 1)).  In bytes: 4x = 4t + 4a_c (one
+            // v_lshl_add with the scalar 4a_c), 4(V - x) = 4V - 4x; a_c is a
+            // scalar running sum (a_{c+1} = a_c + ones_c - 32), so a slot costs
+            // five vector instructions and its store.  No branch on the
+            // chunk's contents: a test for all-equal chunks costs more (scalar
+            // compare and taken branches) than the vector instructions it
+            // would skip.
             {
-                uint32_t ob = cnt0 + ch0 * 64u - run;
+                uint32_t a4 = (cnt0 + ch0 * 64u - run + 32u * (NS - 1)) * 4u;
+                const uint32_t V4 = vbase4 + cnt0 * 4u;
                 static_for<0, NS>([&](auto ci) {
                     constexpr int c = decltype(ci)::value;
                     const uint64_t m = mk[c];
-                    const uint32_t ones = (uint32_t)__popcll(m);
-                    const uint32_t t = mbcnt(m);  // ones below me
-                    const uint32_t od = ob + t;
-                    const uint32_t zd = __builtin_amdgcn_sad_u16(ln, t, run);
-                    // LDS byte address of my destination in pay[]
-                    const uint32_t addr = (__builtin_amdgcn_inverse_ballot_w64(m) ? od : zd) * 4u;
-                    *(lds_ptr_t<uint32_t>)(size_t)addr = p[c];
-                    run += 64u - ones;
-                    ob += ones;
+                    const uint32_t x4 = mbcnt(m) * 4u + a4;
+                    const uint32_t sel4 = __builtin_amdgcn_inverse_ballot_w64(m) ? x4 : V4 - x4;
+                    *(lds_ptr_t<uint32_t>)(size_t)(sel4 + 128u * (uint32_t)c) = p[c];
+                    a4 += (uint32_t)__popcll(m) * 4u - 128u;
+                    // opaque scalar: keeps the running sum in bytes (left to
+                    // itself the compiler factors the 4 out, one more vector op)
+                    asm volatile("" : "+s"(a4));
                 });
             }
             __syncthreads();
@@ -1214,7 +1263,7 @@ __device__ __forceinline__ void pbwt_target(const TgtDev &T, const bool rev, uin
 // register budget is capped so that VGPRs do not lower the occupancy (512 x 20:
 // three workgroups per CU need at most 85 VGPRs).
 constexpr int pbwt_waves_hint(int NT, int CPW) {
-    const int lds = CPW * NT * 4 + 2 * (NT / 64) * 4;
+    const int lds = (int)pbwt_pay_bias(CPW) + CPW * NT * 4 + 2 * (NT / 64) * 4;
     const int w = (163840 / lds) * (NT / 256);
     return w < 1 ? 1 : (w > 6 ? 6 : w);
 }
@@ -1893,7 +1942,8 @@ size_t pbwt_lds_bytes(int v, uint32_t Kpad) {
     const int ns = pbwt_slots_for(Kpad, nt);
     if (ns < 0 || ns > kPbwtVariants[v].cpw) return ~(size_t)0;
     const size_t npos = (size_t)ns * nt;  // NS x NW x 64
-    return npos * 4 + 2 * (nt / 64) * 4;  // payloads (the cref stage shares them), wave counts
+    // bias, payloads (the cref stage shares them), wave counts
+    return pbwt_pay_bias(ns) + npos * 4 + 2 * (nt / 64) * 4;
 }
 hipError_t pbwt_prepare(int v, size_t lds) {
     hipError_t e = pbwt_lds_at_zero((const void *)kPbwtVariants[v].fn);
@@ -1924,7 +1974,7 @@ hipError_t launch_panel_pbwt(const PanelDev &P, uint64_t m0, uint32_t nsites, ui
     const uint32_t nw = (P.N_ad + 63) / 64;
     hipLaunchKernelGGL(k_site_blocks, dim3((nsites + 15) / 16, (nw + 255) / 256), dim3(256), 0, s, P,
                        m0, nsites, blocks, T.Kpad);
-    const size_t lds = (size_t)ns * NT * 4 + 2 * (NT / 64) * 4;
+    const size_t lds = pbwt_pay_bias(ns) + (size_t)ns * NT * 4 + 2 * (NT / 64) * 4;
     hipError_t e = pbwt_lds_at_zero((const void *)k_panel_pbwt<NT, CPW>);
     if (e != hipSuccess) return e;
     e = raise_lds_limit((const void *)k_panel_pbwt<NT, CPW>, lds);
