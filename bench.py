@@ -481,13 +481,18 @@ def batch_bench(args) -> dict:
             nt = min(T, max(args.tune_batches))
             cands = [b for b in args.tune_batches if b <= nt] or [nt]
             tuned = tune(eng, tg, list(range(nt)), batches=cands, use_cache=False, log=log,
-                         layout=args.layout)
+                         layout=args.layout, contexts=args.contexts)
             if not args.batch:
                 batch = tuned["batch"]
             log(f"autotuner: variant {tuned['variant']} ({tuned['threads']} threads x "
                 f"{tuned['chunks_per_wave']} chunks/wave), {batch} targets per submission")
+        from eagleimp_amd.tune import pool_budget
+        maxW = int(tg.W.max())
+        maxK = int(min(int(tg.K.max()), panel.N))
         free, total = eng.mem_info()
-        budget = max(free - (24 << 30), 1 << 30)
+        # HBM for the output pools: free memory less the contexts' own buffers
+        # (measured on a probe context) and a 4 GiB margin
+        budget = pool_budget(eng, args.contexts, batch, maxW, maxK, args.layout)
         nctx = args.contexts
         if tuned is not None and tuned.get("bytes_per_target"):
             # as many contexts as hold a full submission each
@@ -497,8 +502,6 @@ def batch_bench(args) -> dict:
         # is known), the contexts may share that pool: builds take turns and
         # each preparation runs beside the other context's build
         can_share = args.share_pool != "off" and args.contexts > 1 and (tuned is None or nctx == 1)
-        maxW = int(tg.W.max())
-        maxK = int(min(int(tg.K.max()), panel.N))
 
         def pipeline(sts, steps, first):
             """steps batches over the contexts, round robin; deferred targets are

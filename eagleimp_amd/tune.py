@@ -60,18 +60,40 @@ def region_bytes(Mp, K, layout="gpu"):
     return up((Mp + 15) // 16 * kpad * 2) + up(Mp * 4) + up(2 * Mp * rs * 4)
 
 
+def context_bytes(eng: Engine, batch: int, Wmax: int, Kmax: int, layout: str = "gpu") -> int:
+    """Device bytes one stream context holds besides its output pool: free
+    memory before and after making a context with a 1 MiB pool."""
+    f0, _ = eng.mem_info()
+    st = eng.stream(batch, Wmax, Kmax, 1 << 20, timed=True, graph=False, digest=False,
+                    **({"compact": True} if layout == "compact" else {}))
+    f1, _ = eng.mem_info()
+    st.close()
+    return max(int(f0) - int(f1) - (1 << 20), 0)
+
+
+def pool_budget(eng: Engine, contexts: int, batch: int, Wmax: int, Kmax: int,
+                layout: str = "gpu", margin: int = 4 << 30) -> int:
+    """Device bytes left for output pools once `contexts` contexts' own
+    buffers (measured) and a margin are set aside."""
+    ctx = context_bytes(eng, batch, Wmax, Kmax, layout)
+    free, _ = eng.mem_info()
+    return max(int(free) - contexts * ctx - margin, 1 << 28)
+
+
 def candidates(Kmax: int):
     """PBWT variants whose capacity holds K rounded up to 64."""
     kpad = (Kmax + 63) // 64 * 64
     return [v for v in pbwt_variants() if v["max_k"] >= kpad]
 
 
-def tune(eng: Engine, targets, idx, batches=(128, 256), reserve_bytes: int = 24 << 30,
-         use_cache: bool = True, log=None, layout: str = "gpu") -> dict:
+def tune(eng: Engine, targets, idx, batches=(128, 256), reserve_bytes: int | None = None,
+         use_cache: bool = True, log=None, layout: str = "gpu", contexts: int = 2) -> dict:
     """Times every (variant, batch) candidate on the sample targets[idx] and
     sets the fastest variant on the engine.  Returns the choice and the table
     of per-target device milliseconds.  layout: the contexts' output rows,
-    "gpu" (drop-in) or "compact"."""
+    "gpu" (drop-in) or "compact".  The trial pool is what `contexts` contexts
+    sharing one pool would get (pool_budget), or free memory less
+    reserve_bytes when that is given."""
     if layout not in ("gpu", "compact"):
         raise ValueError(f"layout {layout!r}: 'gpu' or 'compact'")
     idx = list(idx)
@@ -88,10 +110,16 @@ def tune(eng: Engine, targets, idx, batches=(128, 256), reserve_bytes: int = 24 
         eng.set_pbwt_variant(best["variant"])
         best = dict(best, cached=True)
         return best
-    free, _ = eng.mem_info()
-    pool = max(free - reserve_bytes, 1 << 28)
+    def trial_pool(b):
+        # what the pool would be with b-target contexts (their own buffers grow with b)
+        if reserve_bytes is None:
+            return pool_budget(eng, contexts, b, Wmax, Kmax, layout)
+        free, _ = eng.mem_info()
+        return max(free - reserve_bytes, 1 << 28)
+
     table = []
     cap = len(idx)  # targets one context's pool holds (learnt from deferrals)
+    estimated = False
     for v in vs:
         eng.set_pbwt_variant(v["variant"])
         todo, seen = list(batches), set()
@@ -100,6 +128,7 @@ def tune(eng: Engine, targets, idx, batches=(128, 256), reserve_bytes: int = 24 
             if b in seen:
                 continue
             seen.add(b)
+            pool = trial_pool(b)
             st = eng.stream(b, Wmax, Kmax, pool, timed=True, graph=False, digest=False,
                             **({"compact": True} if layout == "compact" else {}))
             sub = idx[:b]
@@ -124,7 +153,15 @@ def tune(eng: Engine, targets, idx, batches=(128, 256), reserve_bytes: int = 24 
             if log:
                 log(f"tune: {row}")
             if 0 < done < b:  # the pool holds fewer: measure at its capacity instead
+                # (a smaller submission's contexts leave more for the pool: the
+                # first capacity is estimated from the targets' sizes; if that
+                # still defers, what it placed is the capacity)
                 cap = done
+                if not estimated:
+                    per = float(region_bytes(res.Mp[fit], res.K[fit], layout).max())
+                    if per > 0:
+                        cap = max(done, min(int(trial_pool(done) // per), b - 1))
+                    estimated = True
                 todo.insert(0, cap)
     # submissions that deferred targets did their preparation twice: out
     fitting = [r for r in table if r["deferred"] == 0] or table

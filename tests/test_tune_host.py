@@ -101,3 +101,16 @@ def test_batches_capped_at_pool_capacity(tmp_path, monkeypatch):
     assert best["batch"] == 150 and best["variant"] == 3
     assert all(r["batch"] <= 150 or r["deferred"] > 0 for r in best["table"])
     assert {r["batch"] for r in best["table"] if r["deferred"] == 0} == {150}
+
+
+def test_capacity_estimate_tried_once(tmp_path, monkeypatch):
+    """After the first deferral the tuner estimates the capacity of the
+    smaller submission's pool from the targets' sizes; when that estimate
+    still defers, what it placed is taken: two trials, not a descent."""
+    monkeypatch.setattr(tune, "CACHE", str(tmp_path / "tune.json"))
+    eng = _Engine(cap=150, per={2: 1.0})
+    monkeypatch.setattr(tune, "candidates", lambda K: [dict(variant=2, threads=512,
+                                                            chunks_per_wave=20, max_k=10240)])
+    best = tune.tune(eng, _Targets(300, 10000), range(300), batches=(256,), use_cache=False)
+    assert [r["batch"] for r in best["table"]] == [256, 255, 150]
+    assert best["batch"] == 150 and eng.submits == 3
