@@ -275,15 +275,26 @@ def find_traffic(kernel_name, workload, targets_per_step, layout="gpu"):
     passes (FETCH_SIZE, WRITE_SIZE) on this workload, submission size and
     output layout."""
     import glob
+    near = None
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*traffic*.json"))):
         try:
             tr = json.load(open(path))
         except (OSError, ValueError):
             continue
-        if (tr.get("kernel") == kernel_name and tr.get("workload") == workload
-                and tr.get("targets_per_step") == targets_per_step
-                and tr.get("layout", "gpu") == layout):
+        if (tr.get("kernel") != kernel_name or tr.get("workload") != workload
+                or tr.get("layout", "gpu") != layout):
+            continue
+        if tr.get("targets_per_step") == targets_per_step:
             return round(float(tr["traffic_bytes_per_launch"])), os.path.relpath(path, ROOT)
+        near = near or (tr, path)
+    if near:
+        # a launch of another submission size on the same workload: its bytes
+        # per target (the same targets' rows and cref blocks), scaled
+        tr, path = near
+        n = int(tr["targets_per_step"])
+        return (round(float(tr["traffic_bytes_per_launch"]) * targets_per_step / n),
+                f"{os.path.relpath(path, ROOT)} (measured at {n} targets per launch, "
+                f"scaled per target)")
     return None, None
 
 
@@ -810,11 +821,12 @@ def batch_bench(args) -> dict:
             "traffic_over_alg": None if traffic is None else round(traffic / balg_launch, 3),
             "traffic_over_kernel_bytes": None if traffic is None else round(traffic / kbytes_launch, 3),
             "limiter": None if sq is None else (
-                f"instruction issue: {sq['valu_per_wave_step']} VALU + "
-                f"{sq['salu_per_wave_step']} SALU + {sq['lds_per_wave_step']} LDS instructions "
-                f"per wave-step, a step takes {sq.get('issue_model_ratio', float('nan'))} x "
-                f"issuing them one at a time at 4 cycles each; VALU busy "
-                f"{100 * sq['valu_busy']:.1f}%"),
+                f"vector issue and the step's barrier/LDS latency: "
+                f"{sq['valu_per_wave_step']} VALU + {sq['salu_per_wave_step']} SALU + "
+                f"{sq['lds_per_wave_step']} LDS instructions per wave-step, VALU busy "
+                f"{100 * sq['valu_busy']:.1f}%; a step takes "
+                f"{sq.get('issue_model_ratio', float('nan'))} x issuing all of them one at a "
+                f"time at 4 cycles each"),
             "sq": None if sq is None else {k: sq[k] for k in sq if k not in ("kernel", "workload")},
         },
         "kernel_ms": {k: round(v[0], 3) for k, v in tim.items()},
