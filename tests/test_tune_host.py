@@ -114,3 +114,33 @@ def test_capacity_estimate_tried_once(tmp_path, monkeypatch):
     best = tune.tune(eng, _Targets(300, 10000), range(300), batches=(256,), use_cache=False)
     assert [r["batch"] for r in best["table"]] == [256, 255, 150]
     assert best["batch"] == 150 and eng.submits == 3
+
+
+def test_pool_budget_sets_aside_contexts_and_margin():
+    """pool_budget: free memory less `contexts` times one context's own
+    buffers (measured as the drop in free memory when a context with a 1 MiB
+    pool is made) and the margin."""
+    GiB = 1 << 30
+
+    class _MemEngine(_Engine):
+        def __init__(self, ctx_bytes):
+            super().__init__()
+            self.free, self.ctx = 200 * GiB, ctx_bytes
+
+        def mem_info(self):
+            return self.free, 288 * GiB
+
+        def stream(self, b, W, K, pool, timed=False, graph=True, digest=True):
+            eng = self
+            eng.free -= self.ctx + pool
+
+            class _S(_Stream):
+                def close(s):
+                    eng.free += eng.ctx + pool
+            return _S(self, b)
+
+    eng = _MemEngine(3 * GiB)
+    assert tune.context_bytes(eng, 124, 500000, 10000) == 3 * GiB
+    assert eng.free == 200 * GiB  # the probe context is closed again
+    assert tune.pool_budget(eng, 2, 124, 500000, 10000) == 200 * GiB - 2 * 3 * GiB - 4 * GiB
+    assert tune.pool_budget(eng, 1, 124, 500000, 10000, margin=0) == 197 * GiB
