@@ -901,10 +901,10 @@ __global__ __launch_bounds__(CREF_NT) void k_cref_ident(PanelDev P, const TgtDev
 //           output lane c; the wave's zero count (scalar) to LDS
 //   (barrier) zeros before the wave's first chunk and in the row (count0)
 //           from the NW wave counts, in scalar registers
-//   pass 2  per slot: zeros / ones before the chunk are scalar running sums,
-//           so mbcnt of the ballot (or its complement) with that base is the
-//           destination under the partition; payloads scattered into LDS (two
-//           contiguous runs per chunk); output lanes store their data /
+//   pass 2  per slot: the ones before the chunk are a scalar running sum, so
+//           mbcnt of the ballot plus that base gives a one's destination and
+//           a lane constant minus it a zero's; payloads scattered into LDS
+//           (two contiguous runs per chunk); output lanes store their data /
 //           cumulative-zero pairs
 //   (barrier)
 //   pass 3  read back the payloads of my positions.
@@ -1161,9 +1161,10 @@ __device__ __forceinline__ void pbwt_steps(const TgtDev &T, const bool rev, uint
             // V = cnt0 + 64 ch0 + ln + 64 (NS - 1) the same for every slot, so
             //   PBIAS + 4 dest = 4 sel + 128 c,  sel = bit ? x : V - x >= 0,
             // and 128 c is the store's immediate offset (pay[] starts at byte
-            // PBIAS = 128 (NS - 1)).  In bytes: 4x = 4t + 4a_c (one
-            // v_lshl_add with the scalar 4a_c), 4(V - x) = 4V - 4x; a_c is a
-            // scalar running sum (a_{c+1} = a_c + ones_c - 32), so a slot costs
+            // PBIAS = 128 (NS - 1)).  In bytes: 4x = 4t + 4a_c with
+            // a_c = ob_c + 32 (NS - 1 - c) (one v_lshl_add with the scalar
+            // 4a_c), 4(V - x) = 4V - 4x; a_c is a scalar running sum
+            // (a_{c+1} = a_c + ones_c - 32), so a slot costs
             // five vector instructions and its store.  No branch on the
             // chunk's contents: a test for all-equal chunks costs more (scalar
             // compare and taken branches) than the vector instructions it
