@@ -91,7 +91,7 @@ struct BatchDev {
 };
 
 // 16-row blocks per work item of the subset cref kernel (256 splits)
-constexpr uint32_t kCrefGroup = 32;
+constexpr uint32_t kCrefGroup = 128;
 
 // cref work item: target t, blocks [b0, b1)
 struct CrefItem {

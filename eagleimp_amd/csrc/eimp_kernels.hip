@@ -732,29 +732,46 @@ __device__ __forceinline__ void cref_range_hap(const PanelDev &P, const TgtDev &
         const uint32_t h = act ? (ident ? k : bh[k]) : 0u;
         const gptr_t<const uint32_t> row = gmem((const uint32_t *)P.hap) + (uint64_t)h * rowlen;
         uint32_t cin = 0, acc = 0, j2 = 0, b = b0;  // j2: bit position of the next pair
-        // row word x (x <= i1, inside the row) of window word x
-        auto rword = [&](uint32_t x) -> uint32_t {
-            const uint64_t q = q0 + x;
-            return (x <= i1 && q < rowlen) ? row[q] : 0u;
-        };
-        // the row words of eight window words are loaded together, one batch
-        // ahead of the words being processed
-        uint32_t w8 = rword(i0), nx[8];
+        // The row is read in chunks of 16 words (64 bytes) aligned in the row
+        // (rows start on 64-byte boundaries), each as four 16-byte loads, one
+        // chunk ahead of the words being processed: a lane takes a cache
+        // line's words in two bursts instead of four, so fewer lines are
+        // fetched again after the cache has dropped them.  Window word i is
+        // bits sh.. of row words q0 + i and q0 + i + 1.
+        auto chunk = [&](uint64_t R, uint32_t (&c)[16]) {
+            if (R + 16 <= rowlen) {
+                const gptr_t<const u32x4> r4 = (gptr_t<const u32x4>)(row + R);
 #pragma unroll
-        for (int u = 0; u < 8; u++) nx[u] = rword(i0 + 1 + (uint32_t)u);
-        for (uint32_t ib = i0; ib < i1; ib += 8) {
-            uint32_t w[9];
-            w[0] = w8;
+                for (int u = 0; u < 4; u++) {
+                    const u32x4 x = r4[u];
+                    c[4 * u] = x[0];
+                    c[4 * u + 1] = x[1];
+                    c[4 * u + 2] = x[2];
+                    c[4 * u + 3] = x[3];
+                }
+            } else {
 #pragma unroll
-            for (int u = 0; u < 8; u++) w[u + 1] = nx[u];
-            w8 = w[8];
-            if (ib + 8 < i1) {
-#pragma unroll
-                for (int u = 0; u < 8; u++) nx[u] = rword(ib + 9 + (uint32_t)u);
+                for (int u = 0; u < 16; u++) c[u] = R + u < rowlen ? row[R + u] : 0u;
             }
+        };
+        const uint64_t R0 = (q0 + i0) & ~15ull;  // the first chunk
+        uint32_t cur[16], nx[16];
+        chunk(R0, nx);
+        for (uint64_t R = R0; R < q0 + i1; R += 16) {
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const uint32_t i = ib + (uint32_t)u;
+            for (int u = 0; u < 16; u++) cur[u] = nx[u];
+            chunk(R + 16, nx);
+            uint32_t w[17];
+#pragma unroll
+            for (int u = 0; u < 16; u++) w[u] = cur[u];
+            w[16] = nx[0];
+            // window word of the chunk's first word (below i0 for the first
+            // chunk: those words are skipped)
+            const int64_t ib = (int64_t)R - (int64_t)q0;
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+                if (ib + u < (int64_t)i0) continue;
+                const uint32_t i = (uint32_t)(ib + u);
                 if (i >= i1) break;
                 const uint32_t v = __builtin_amdgcn_alignbit(w[u + 1], w[u], sh);
                 uint32_t rng = ~0u;
