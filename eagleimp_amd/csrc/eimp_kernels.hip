@@ -755,12 +755,16 @@ __device__ __forceinline__ void cref_range_hap(const PanelDev &P, const TgtDev &
             }
         };
         const uint64_t R0 = (q0 + i0) & ~15ull;  // the first chunk
-        uint32_t cur[16], nx[16];
+        uint32_t cur[16], nx[16], nx2[16];
         chunk(R0, nx);
+        chunk(R0 + 16, nx2);
         for (uint64_t R = R0; R < q0 + i1; R += 16) {
 #pragma unroll
-            for (int u = 0; u < 16; u++) cur[u] = nx[u];
-            chunk(R + 16, nx);
+            for (int u = 0; u < 16; u++) {
+                cur[u] = nx[u];
+                nx[u] = nx2[u];
+            }
+            chunk(R + 32, nx2);
             uint32_t w[17];
 #pragma unroll
             for (int u = 0; u < 16; u++) w[u] = cur[u];
