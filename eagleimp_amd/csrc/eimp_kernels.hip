@@ -733,9 +733,9 @@ __device__ __forceinline__ void cref_range_hap(const PanelDev &P, const TgtDev &
         const gptr_t<const uint32_t> row = gmem((const uint32_t *)P.hap) + (uint64_t)h * rowlen;
         uint32_t cin = 0, acc = 0, j2 = 0, b = b0;  // j2: bit position of the next pair
         // The row is read in chunks of 16 words (64 bytes) aligned in the row
-        // (rows start on 64-byte boundaries), each as four 16-byte loads, one
-        // chunk ahead of the words being processed: a lane takes a cache
-        // line's words in two bursts instead of four, so fewer lines are
+        // (rows start on 64-byte boundaries), each as four 16-byte loads, two
+        // chunks ahead of the words being processed: a lane asks for both
+        // halves of a 128-byte line close together, so fewer lines are
         // fetched again after the cache has dropped them.  Window word i is
         // bits sh.. of row words q0 + i and q0 + i + 1.
         auto chunk = [&](uint64_t R, uint32_t (&c)[16]) {
